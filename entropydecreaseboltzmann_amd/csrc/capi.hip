@@ -1,0 +1,1088 @@
+// capi.hip -- the C ABI (include/edb.h): QCalculator, fft3d, EntropyFix and
+// the BKW3D drivers, as host C++ over the HIP kernels of this directory.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/edb.h"
+#include "edb_internal.hpp"
+#include "ops.hpp"
+
+namespace edb {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+namespace {
+
+template <typename T>
+int dmalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return kOk;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    set_error(std::string("hipMalloc(") + std::to_string(sizeof(T) * count) + " bytes): " +
+              hipGetErrorString(e));
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? kNoMemory : kHipError;
+  }
+  return kOk;
+}
+
+template <typename T>
+void dfree(T** p) {
+  if (*p) (void)hipFree((void*)*p);
+  *p = nullptr;
+}
+
+int pad64(int b) { return ((b + 63) / 64) * 64; }
+
+int cu_count() {
+  static int cached = 0;
+  if (cached) return cached;
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+    n = 256;
+  cached = n;
+  return n;
+}
+
+struct PrefixEntry {
+  long* d_prefix = nullptr;
+  int nitems = 0;
+  long total = 0;
+};
+
+}  // namespace
+}  // namespace edb
+
+using namespace edb;
+
+struct edb_qcalc {
+  int N = 0, alpha = 0, filter = 1;
+  double L = 0.;
+  Geometry g;
+  hipStream_t stream = nullptr;
+  BetaTables tables;
+  bool tables_ready = false;
+  // plain FP64
+  double* d_plain = nullptr;
+  int ntile = 0;
+  int nslab = 16;
+  double* d_dup = nullptr;
+  double* d_part = nullptr;
+  // paired FP32
+  float* d_pair = nullptr;
+  int4* d_rowinfo = nullptr;
+  uint32_t* d_sched = nullptr;
+  std::vector<int4> rowinfo;
+  int nrows_half = 0;
+  long nblocks = 0;
+  std::map<std::pair<int, int>, PrefixEntry> prefixes;
+  double* d_slots = nullptr;
+  size_t slots_count = 0;
+  int nwaves = 0;
+  std::mutex mu;
+};
+
+namespace {
+
+int ensure_tables(edb_qcalc* q) {
+  if (q->tables_ready) return kOk;
+  EDB_TRY(beta_tables_init(&q->tables, q->N, q->alpha, q->L, q->filter));
+  q->tables_ready = true;
+  return kOk;
+}
+
+int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out) {
+  auto key = std::make_pair(groups, hermitian);
+  auto it = q->prefixes.find(key);
+  if (it == q->prefixes.end()) {
+    int nrows = hermitian ? q->nrows_half : (int)q->rowinfo.size();
+    std::vector<long> pre;
+    batched_prefix(q->rowinfo, nrows, groups, batched_ksplit(q->N), &pre);
+    PrefixEntry e;
+    e.nitems = (int)pre.size() - 1;
+    e.total = pre.back();
+    EDB_TRY(dmalloc(&e.d_prefix, pre.size()));
+    EDB_HIP_TRY(hipMemcpy(e.d_prefix, pre.data(), sizeof(long) * pre.size(), hipMemcpyHostToDevice));
+    it = q->prefixes.emplace(key, e).first;
+  }
+  *out = &it->second;
+  return kOk;
+}
+
+int ensure_slots(edb_qcalc* q, int nwaves) {
+  const int ks = batched_ksplit(q->N);
+  const int kpt = (q->g.M + ks - 1) / ks;
+  size_t need = (size_t)nwaves * 2 * kpt * 64 * 2;
+  if (need <= q->slots_count) return kOk;
+  dfree(&q->d_slots);
+  q->slots_count = 0;
+  EDB_TRY(dmalloc(&q->d_slots, need));
+  q->slots_count = need;
+  return kOk;
+}
+
+// Batched Q on device arrays already in the kernel layout.
+int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, const float* d_fin,
+                   hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+  EDB_REQUIRE(q->d_pair, "calQ_batched: genB_batched has not run");
+  EDB_REQUIRE(batch >= 1 && ld % 64 == 0 && batch <= ld, "calQ_batched: bad batch / ld");
+  const int groups = (batch + 63) / 64;
+  PrefixEntry* pe = nullptr;
+  EDB_TRY(get_prefix(q, groups, hermitian ? 1 : 0, &pe));
+  int nwaves = q->nwaves > 0 ? q->nwaves : cu_count() * 8;
+  if (nwaves > pe->total) nwaves = (int)pe->total;
+  EDB_TRY(ensure_slots(q, nwaves));
+  return launch_calq_batched_impl(q->N, q->d_pair, q->d_sched, q->d_rowinfo, pe->d_prefix, pe->nitems,
+                                  pe->total, d_fin, ld, groups, hermitian ? 1 : 0, d_q, q->d_slots,
+                                  nwaves, s, ev0, ev1);
+}
+
+int ensure_plain_scratch(edb_qcalc* q) {
+  if (q->d_dup) return kOk;
+  long W = 2 * q->g.M - 1;
+  EDB_TRY(dmalloc(&q->d_dup, (size_t)(W * W * W * 2)));
+  EDB_TRY(dmalloc(&q->d_part, (size_t)(q->nslab * q->g.NQ * 2)));
+  return kOk;
+}
+
+// Plain FP64 Q of one column: d_fc FP64 centred [M^3] with stride fc_stride
+// (complex elements), result centred into d_q with stride q_stride.
+int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, int q_stride,
+                 hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+  EDB_REQUIRE(q->d_plain, "calQ: genB has not run");
+  EDB_TRY(ensure_plain_scratch(q));
+  EDB_TRY(launch_build_dup(q->N, d_fc, fc_stride, q->d_dup, s));
+  if (ev0) EDB_HIP_TRY(hipEventRecord(ev0, s));
+  EDB_TRY(launch_calq_plain64(q->N, q->d_plain, q->d_dup, d_fc, fc_stride, q->d_part, q->nslab, s));
+  if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
+  EDB_TRY(launch_reduce_plain(q->N, q->d_part, q->nslab, d_q, q_stride, s));
+  return kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* edb_last_error(void) { return edb::last_error(); }
+
+int edb_version(int* major, int* minor) {
+  if (major) *major = 0;
+  if (minor) *minor = 1;
+  return EDB_OK;
+}
+
+int edb_device_count(int* count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return EDB_OK;
+}
+
+int edb_sync(void) {
+  EDB_HIP_TRY(hipDeviceSynchronize());
+  return EDB_OK;
+}
+
+// ------------------------------------------------------------ QCalculator
+
+int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out) {
+  EDB_REQUIRE(out, "edb_qcalc_create: null output");
+  *out = nullptr;
+  EDB_REQUIRE(N >= 3 && N <= kMaxNCalq, "edb_qcalc_create: N must be in [3, 33]");
+  EDB_REQUIRE(alpha == 0 || alpha == 1, "edb_qcalc_create: alpha must be 0 or 1");
+  EDB_REQUIRE(L > 0. && std::isfinite(L), "edb_qcalc_create: L must be positive");
+  edb_qcalc* q = new edb_qcalc();
+  q->N = N;
+  q->alpha = alpha;
+  q->L = L;
+  q->g = Geometry(N);
+  q->ntile = (int)((q->g.NQ + 63) / 64);
+  // enough l-slabs to give every SIMD several waves of the HBM stream
+  q->nslab = std::max(1, std::min(64, (cu_count() * 16 + q->ntile - 1) / q->ntile));
+  hipError_t e = hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    delete q;
+    return EDB_EHIP;
+  }
+  *out = q;
+  return EDB_OK;
+}
+
+int edb_qcalc_close_filter(edb_qcalc* q) {
+  EDB_REQUIRE(q, "null handle");
+  EDB_REQUIRE(!q->d_plain && !q->d_pair, "closeFilter must precede genB");
+  q->filter = 0;
+  if (q->tables_ready) {
+    beta_tables_free(&q->tables);
+    q->tables_ready = false;
+  }
+  return EDB_OK;
+}
+
+int edb_qcalc_genB(edb_qcalc* q) {
+  EDB_REQUIRE(q, "null handle");
+  std::lock_guard<std::mutex> lk(q->mu);
+  if (q->d_plain) return EDB_OK;
+  EDB_TRY(ensure_tables(q));
+  EDB_TRY(dmalloc(&q->d_plain, (size_t)q->ntile * q->g.NQ * 64));
+  EDB_TRY(launch_genb_plain64(q->tables, q->d_plain, q->ntile, q->stream));
+  EDB_HIP_TRY(hipStreamSynchronize(q->stream));
+  return EDB_OK;
+}
+
+int edb_qcalc_genB_batched(edb_qcalc* q) {
+  EDB_REQUIRE(q, "null handle");
+  std::lock_guard<std::mutex> lk(q->mu);
+  if (q->d_pair) return EDB_OK;
+  if (!batched_supported(q->N)) {
+    set_error("genB_batched: no compiled batched kernel for N = " + std::to_string(q->N));
+    return EDB_EUNSUPPORTED;
+  }
+  EDB_TRY(ensure_tables(q));
+  std::vector<uint32_t> sched;
+  build_pair_plan(q->N, &q->rowinfo, &sched, &q->nrows_half);
+  q->nblocks = (long)sched.size();
+  EDB_TRY(dmalloc(&q->d_rowinfo, q->rowinfo.size()));
+  EDB_TRY(dmalloc(&q->d_sched, sched.size()));
+  EDB_HIP_TRY(hipMemcpy(q->d_rowinfo, q->rowinfo.data(), sizeof(int4) * q->rowinfo.size(),
+                        hipMemcpyHostToDevice));
+  EDB_HIP_TRY(hipMemcpy(q->d_sched, sched.data(), sizeof(uint32_t) * sched.size(), hipMemcpyHostToDevice));
+  EDB_TRY(dmalloc(&q->d_pair, (size_t)q->nblocks * q->g.M * q->g.M));
+  EDB_TRY(launch_genb_pair32(q->tables, (const int*)q->d_rowinfo, q->d_sched, (int)q->rowinfo.size(),
+                             q->nblocks, q->d_pair, q->stream));
+  EDB_HIP_TRY(hipStreamSynchronize(q->stream));
+  return EDB_OK;
+}
+
+int edb_qcalc_calQ_device(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream) {
+  EDB_REQUIRE(q && d_qout && d_fin, "calQ_device: null argument");
+  hipStream_t s = stream ? (hipStream_t)stream : q->stream;
+  double *fc = nullptr, *qc = nullptr;
+  EDB_TRY(dmalloc(&fc, (size_t)q->g.NQ * 2));
+  int st = dmalloc(&qc, (size_t)q->g.NQ * 2);
+  if (st == kOk) st = launch_pack(q->N, 1, 1, false, fc, d_fin, nullptr, 0., s);
+  if (st == kOk) st = plain_column(q, fc, 1, qc, 1, s);
+  if (st == kOk) st = launch_unpack(q->N, 1, 1, d_qout, qc, s);
+  if (st == kOk && hipStreamSynchronize(s) != hipSuccess) {
+    set_error("calQ_device: stream synchronisation failed");
+    st = kHipError;
+  }
+  dfree(&fc);
+  dfree(&qc);
+  return st;
+}
+
+int edb_qcalc_calQ(edb_qcalc* q, double* qout, const double* fin) {
+  EDB_REQUIRE(q && qout && fin, "calQ: null argument");
+  size_t bytes = sizeof(double) * 2 * q->g.N3;
+  double *din = nullptr, *dout = nullptr;
+  EDB_TRY(dmalloc(&din, (size_t)q->g.N3 * 2));
+  int st = dmalloc(&dout, (size_t)q->g.N3 * 2);
+  if (st == kOk && hipMemcpy(din, fin, bytes, hipMemcpyHostToDevice) != hipSuccess) st = kHipError;
+  if (st == kOk) st = edb_qcalc_calQ_device(q, dout, din, q->stream);
+  if (st == kOk && hipMemcpy(qout, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = kHipError;
+  if (st == kHipError && g_err.empty()) set_error("calQ: copy failed");
+  dfree(&din);
+  dfree(&dout);
+  return st;
+}
+
+int edb_qcalc_calQ_batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_qout,
+                                  const float* d_fin, void* stream) {
+  EDB_REQUIRE(q && d_qout && d_fin, "calQ_batched_device: null argument");
+  hipStream_t s = stream ? (hipStream_t)stream : q->stream;
+  return batched_device(q, batch, ld, hermitian, d_qout, d_fin, s);
+}
+
+int edb_qcalc_calQ_batched(edb_qcalc* q, int batch, int hermitian, double* qout, const double* fin) {
+  EDB_REQUIRE(q && qout && fin && batch >= 1, "calQ_batched: bad argument");
+  const int ld = pad64(batch);
+  const long N3 = q->g.N3, NQ = q->g.NQ;
+  hipStream_t s = q->stream;
+  double *dhost = nullptr, *dgrid = nullptr, *dq = nullptr;
+  float* din = nullptr;
+  int st = dmalloc(&dhost, (size_t)N3 * batch * 2);
+  if (st == kOk) st = dmalloc(&dgrid, (size_t)N3 * ld * 2);
+  if (st == kOk) st = dmalloc(&dq, (size_t)NQ * ld * 2);
+  if (st == kOk) st = dmalloc(&din, (size_t)NQ * ld * 2);
+  if (st == kOk &&
+      hipMemcpy(dhost, fin, sizeof(double) * 2 * N3 * batch, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("calQ_batched: host-to-device copy failed");
+    st = kHipError;
+  }
+  if (st == kOk) st = launch_gather_cols(N3, batch, ld, dhost, dgrid, s);
+  if (st == kOk) st = launch_pack(q->N, batch, ld, true, din, dgrid, nullptr, 0., s);
+  if (st == kOk) st = batched_device(q, batch, ld, hermitian, dq, din, s);
+  if (st == kOk) st = launch_unpack(q->N, batch, ld, dgrid, dq, s);
+  if (st == kOk) st = launch_scatter_cols(N3, batch, ld, dgrid, dhost, s);
+  if (st == kOk && hipStreamSynchronize(s) != hipSuccess) {
+    set_error("calQ_batched: stream synchronisation failed");
+    st = kHipError;
+  }
+  if (st == kOk &&
+      hipMemcpy(qout, dhost, sizeof(double) * 2 * N3 * batch, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("calQ_batched: device-to-host copy failed");
+    st = kHipError;
+  }
+  dfree(&dhost);
+  dfree(&dgrid);
+  dfree(&dq);
+  dfree(&din);
+  return st;
+}
+
+int edb_qcalc_copy_B(edb_qcalc* q, double* Bout) {
+  EDB_REQUIRE(q && Bout, "copy_B: null argument");
+  EDB_REQUIRE(q->g.NQ <= 50000, "copy_B: M^6 too large to copy (M <= 36)");
+  EDB_TRY(ensure_tables(q));
+  long total = q->g.NQ * q->g.NQ;
+  double* d = nullptr;
+  EDB_TRY(dmalloc(&d, (size_t)total));
+  int st = launch_genb_ref64(q->tables, d, q->stream);
+  if (st == kOk && hipMemcpyAsync(Bout, d, sizeof(double) * total, hipMemcpyDeviceToHost, q->stream) !=
+                       hipSuccess)
+    st = kHipError;
+  if (st == kOk && hipStreamSynchronize(q->stream) != hipSuccess) st = kHipError;
+  if (st == kHipError) set_error("copy_B: copy failed");
+  dfree(&d);
+  return st;
+}
+
+int edb_qcalc_info(const edb_qcalc* q, int* N, int* M, long* bytes_plain64, long* bytes_pair32) {
+  EDB_REQUIRE(q, "null handle");
+  if (N) *N = q->N;
+  if (M) *M = q->g.M;
+  if (bytes_plain64) *bytes_plain64 = (long)q->ntile * q->g.NQ * 64 * 8;
+  if (bytes_pair32) {
+    if (q->d_pair) {
+      *bytes_pair32 = q->nblocks * q->g.M * q->g.M * 4;
+    } else {
+      std::vector<int4> ri;
+      std::vector<uint32_t> sc;
+      int nh = 0;
+      build_pair_plan(q->N, &ri, &sc, &nh);
+      *bytes_pair32 = (long)sc.size() * q->g.M * q->g.M * 4;
+    }
+  }
+  return EDB_OK;
+}
+
+int edb_qcalc_time_batched(edb_qcalc* q, int batch, int hermitian, int reps, double* ms) {
+  EDB_REQUIRE(q && ms && reps >= 1 && batch >= 1, "time_batched: bad argument");
+  const int ld = pad64(batch);
+  const long NQ = q->g.NQ;
+  float* din = nullptr;
+  double* dq = nullptr;
+  EDB_TRY(dmalloc(&din, (size_t)NQ * ld * 2));
+  int st = dmalloc(&dq, (size_t)NQ * ld * 2);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (st == kOk) {
+    std::vector<float> h((size_t)NQ * ld * 2);
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    for (auto& v : h) {
+      x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+      v = (float)((double)(x >> 11) / 9007199254740992.0 - 0.5);
+    }
+    if (hipMemcpy(din, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+      st = kHipError;
+  }
+  if (st == kOk && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = kHipError;
+  if (st == kOk) st = batched_device(q, batch, ld, hermitian, dq, din, q->stream);  // warm-up
+  if (st == kOk && hipEventRecord(e0, q->stream) != hipSuccess) st = kHipError;
+  for (int r = 0; st == kOk && r < reps; ++r) st = batched_device(q, batch, ld, hermitian, dq, din, q->stream);
+  if (st == kOk && hipEventRecord(e1, q->stream) != hipSuccess) st = kHipError;
+  if (st == kOk && hipEventSynchronize(e1) != hipSuccess) st = kHipError;
+  float el = 0.f;
+  if (st == kOk && hipEventElapsedTime(&el, e0, e1) != hipSuccess) st = kHipError;
+  if (st == kOk) *ms = el / reps;
+  if (st == kHipError) set_error("time_batched: HIP failure");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dfree(&din);
+  dfree(&dq);
+  return st;
+}
+
+int edb_qcalc_time_plain(edb_qcalc* q, int reps, double* ms) {
+  EDB_REQUIRE(q && ms && reps >= 1, "time_plain: bad argument");
+  const long NQ = q->g.NQ;
+  double *fc = nullptr, *dq = nullptr;
+  EDB_TRY(dmalloc(&fc, (size_t)NQ * 2));
+  int st = dmalloc(&dq, (size_t)NQ * 2);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (st == kOk) {
+    std::vector<double> h((size_t)NQ * 2);
+    for (size_t i = 0; i < h.size(); ++i) h[i] = std::sin(0.37 * (double)i);
+    if (hipMemcpy(fc, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+      st = kHipError;
+  }
+  if (st == kOk && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = kHipError;
+  if (st == kOk) st = plain_column(q, fc, 1, dq, 1, q->stream);
+  if (st == kOk && hipEventRecord(e0, q->stream) != hipSuccess) st = kHipError;
+  for (int r = 0; st == kOk && r < reps; ++r) st = plain_column(q, fc, 1, dq, 1, q->stream);
+  if (st == kOk && hipEventRecord(e1, q->stream) != hipSuccess) st = kHipError;
+  if (st == kOk && hipEventSynchronize(e1) != hipSuccess) st = kHipError;
+  float el = 0.f;
+  if (st == kOk && hipEventElapsedTime(&el, e0, e1) != hipSuccess) st = kHipError;
+  if (st == kOk) *ms = el / reps;
+  if (st == kHipError) set_error("time_plain: HIP failure");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dfree(&fc);
+  dfree(&dq);
+  return st;
+}
+
+int edb_qcalc_destroy(edb_qcalc* q) {
+  if (!q) return EDB_OK;
+  if (q->stream) (void)hipStreamSynchronize(q->stream);
+  dfree(&q->d_plain);
+  dfree(&q->d_dup);
+  dfree(&q->d_part);
+  dfree(&q->d_pair);
+  dfree(&q->d_rowinfo);
+  dfree(&q->d_sched);
+  dfree(&q->d_slots);
+  for (auto& kv : q->prefixes) dfree(&kv.second.d_prefix);
+  if (q->tables_ready) beta_tables_free(&q->tables);
+  if (q->stream) (void)hipStreamDestroy(q->stream);
+  delete q;
+  return EDB_OK;
+}
+
+// ----------------------------------------------------------------- fft3d
+
+static int fft_host(int Nx, int Ny, int Nz, const double* in, double* out, int sign, double scale) {
+  EDB_REQUIRE(in && out, "fft3d: null argument");
+  EDB_REQUIRE(Nx >= 1 && Ny >= 1 && Nz >= 1 && Nx <= kMaxN && Ny <= kMaxN && Nz <= kMaxN,
+              "fft3d: lengths must be in [1, 64]");
+  long n = (long)Nx * Ny * Nz;
+  const int ld = 64;
+  double *dh = nullptr, *dg = nullptr, *dt = nullptr;
+  int st = dmalloc(&dh, (size_t)n * 2);
+  if (st == kOk) st = dmalloc(&dg, (size_t)n * ld * 2);
+  if (st == kOk) st = dmalloc(&dt, (size_t)n * ld * 2);
+  hipStream_t s = nullptr;
+  if (st == kOk && hipMemcpy(dh, in, sizeof(double) * 2 * n, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("fft3d: copy failed");
+    st = kHipError;
+  }
+  if (st == kOk) st = launch_gather_cols(n, 1, ld, dh, dg, s);
+  if (st == kOk) st = launch_fft3d(Nx, Ny, Nz, 1, ld, sign, scale, dg, dg, dt, s);
+  if (st == kOk) st = launch_scatter_cols(n, 1, ld, dg, dh, s);
+  if (st == kOk && hipMemcpy(out, dh, sizeof(double) * 2 * n, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("fft3d: copy failed");
+    st = kHipError;
+  }
+  dfree(&dh);
+  dfree(&dg);
+  dfree(&dt);
+  return st;
+}
+
+int edb_fft3d_fft(int Nx, int Ny, int Nz, const double* re, double* fre) {
+  return fft_host(Nx, Ny, Nz, re, fre, -1, 1.0);
+}
+
+int edb_fft3d_ifft(int Nx, int Ny, int Nz, double* re, const double* fre) {
+  return fft_host(Nx, Ny, Nz, fre, re, +1, 1. / (double)((long)Nx * Ny * Nz));
+}
+
+int edb_fft3d_batched_device(int Nx, int Ny, int Nz, int batch, int ld, int inverse, const double* d_in,
+                             double* d_out, void* stream) {
+  EDB_REQUIRE(d_in && d_out, "fft3d_batched_device: null argument");
+  double scale = inverse ? 1. / (double)((long)Nx * Ny * Nz) : 1.0;
+  return launch_fft3d(Nx, Ny, Nz, batch, ld, inverse ? +1 : -1, scale, d_in, d_out, nullptr,
+                      (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------ EntropyFix
+
+int edb_efix_init(edb_efix_state* st, long Nt, double eps) {
+  EDB_REQUIRE(st && Nt >= 1, "efix_init: bad argument");
+  st->Nt = Nt;
+  st->eps = eps;
+  st->average = 1.0;  // EntropyFix.h:20
+  st->average_flag = 0;
+  return EDB_OK;
+}
+
+static int efix_run(edb_efix_state* st, int mode, long Nf, double* f_host, bool write_back, double aim,
+                    double* result) {
+  EDB_REQUIRE(st && f_host, "EntropyFix: null argument");
+  EDB_REQUIRE(Nf == st->Nt, "EntropyFix: Nf != Nt (the reference asserts Nf == Nt)");
+  double *df = nullptr, *ds = nullptr;
+  int s = dmalloc(&df, (size_t)Nf);
+  if (s == kOk) s = dmalloc(&ds, 3);
+  double hst[3] = {st->average, (double)st->average_flag, 0.};
+  if (s == kOk && (hipMemcpy(df, f_host, sizeof(double) * Nf, hipMemcpyHostToDevice) != hipSuccess ||
+                   hipMemcpy(ds, hst, sizeof(hst), hipMemcpyHostToDevice) != hipSuccess)) {
+    set_error("EntropyFix: copy failed");
+    s = kHipError;
+  }
+  if (s == kOk) s = launch_efix(mode, Nf, df, st->eps, aim, ds, nullptr);
+  if (s == kOk && hipMemcpy(hst, ds, sizeof(hst), hipMemcpyDeviceToHost) != hipSuccess) s = kHipError;
+  if (s == kOk && write_back &&
+      hipMemcpy(f_host, df, sizeof(double) * Nf, hipMemcpyDeviceToHost) != hipSuccess)
+    s = kHipError;
+  if (s == kOk) {
+    st->average = hst[0];
+    st->average_flag = hst[1] != 0.;
+    if (result) *result = hst[2];
+  } else if (s == kHipError) {
+    set_error("EntropyFix: HIP failure");
+  }
+  dfree(&df);
+  dfree(&ds);
+  return s;
+}
+
+int edb_efix_set_average(edb_efix_state* st, long Nf, const double* f) {
+  return efix_run(st, 0, Nf, const_cast<double*>(f), false, 0., nullptr);
+}
+
+int edb_efix_entropy_cal(edb_efix_state* st, long Nf, const double* f, double* H) {
+  EDB_REQUIRE(H, "EntropyCal: null output");
+  return efix_run(st, 1, Nf, const_cast<double*>(f), false, 0., H);
+}
+
+int edb_efix_fixsol(edb_efix_state* st, long Nf, double* f, double aim) {
+  return efix_run(st, 2, Nf, f, true, aim, nullptr);
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- solver
+
+struct edb_solver {
+  edb_solver_params p{};
+  edb_qcalc* qc = nullptr;
+  bool own_qc = false;
+  Geometry g;
+  int ld = 0;
+  hipStream_t stream = nullptr;
+  double *Fh = nullptr, *Fr = nullptr, *Ft = nullptr, *S1 = nullptr, *S2 = nullptr, *Fold = nullptr;
+  double* Q[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* in = nullptr;
+  double *H0 = nullptr, *avg = nullptr;
+  int *avg_flag = nullptr, *fixed = nullptr;
+  double *hist_H = nullptr, *hist_Hfix = nullptr;
+  int* hist_fixed = nullptr;
+  int* d_rec = nullptr;
+  double *d_out = nullptr;
+  double t = 0.;
+  int ref_step = 0;     // the driver's own step counter
+  int taken = 0;        // steps taken in total
+  int dufort_phase = 0; // 0 start-up RK2 steps, 1 DuFort-Frankel
+  bool ready = false;
+  // graphs keyed by (phase, eligible)
+  bool use_graph = false;
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  // Q-kernel timing
+  bool qtiming = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  size_t ev_used = 0;
+};
+
+namespace {
+
+bool batched(const edb_solver* s) { return s->p.precision == EDB_PREC_FP32_BATCHED; }
+
+int next_events(edb_solver* s, hipEvent_t* e0, hipEvent_t* e1) {
+  *e0 = *e1 = nullptr;
+  if (!s->qtiming || s->use_graph) return kOk;
+  if (s->ev_used == s->ev.size()) {
+    hipEvent_t a, b;
+    EDB_HIP_TRY(hipEventCreate(&a));
+    EDB_HIP_TRY(hipEventCreate(&b));
+    s->ev.push_back({a, b});
+  }
+  *e0 = s->ev[s->ev_used].first;
+  *e1 = s->ev[s->ev_used].second;
+  s->ev_used++;
+  return kOk;
+}
+
+// Q of the stage state X (+ c0*Qprev) into Qout (centred, [M^3][ld]).
+int qeval(edb_solver* s, const double* X, const double* Qprev, double c0, double* Qout) {
+  const int N = s->p.N, batch = s->p.batch, ld = s->ld;
+  EDB_TRY(launch_pack(N, batch, ld, batched(s), s->in, X, Qprev, c0, s->stream));
+  if (batched(s)) {
+    hipEvent_t e0, e1;
+    EDB_TRY(next_events(s, &e0, &e1));
+    return batched_device(s->qc, batch, ld, s->p.hermitian, Qout, (const float*)s->in, s->stream, e0, e1);
+  }
+  for (int i = 0; i < batch; ++i) {
+    hipEvent_t e0, e1;
+    EDB_TRY(next_events(s, &e0, &e1));
+    EDB_TRY(plain_column(s->qc, (const double*)s->in + 2 * i, ld, Qout + 2 * i, ld, s->stream, e0, e1));
+  }
+  return kOk;
+}
+
+int combine(edb_solver* s, int op, double* out, const double* X, const double* Y, const double* Q0,
+            const double* Q1, const double* Q2, const double* Q3, double c0, double c1 = 0.,
+            double c2 = 0.) {
+  return launch_combine(op, s->p.N, s->p.batch, s->ld, out, X, Y, Q0, Q1, Q2, Q3, c0, c1, c2,
+                        s->stream);
+}
+
+// ifft, calEntropy, entropy fix (and the forward transform of fixed
+// instances in applied mode): the tail of every driver's loop body.
+int post_step(edb_solver* s, int eligible) {
+  const int N = s->p.N, batch = s->p.batch, ld = s->ld;
+  const long N3 = s->g.N3;
+  EDB_TRY(launch_fft3d(N, N, N, batch, ld, +1, 1. / (double)N3, s->Fh, s->Fr, s->Ft, s->stream));
+  EntropyFixArgs a;
+  a.N3 = N3;
+  a.ld = ld;
+  a.F = (double2*)s->Fr;
+  a.W = s->S1;
+  a.H0 = s->H0;
+  a.avg = s->avg;
+  a.avg_flag = s->avg_flag;
+  a.fixed = s->fixed;
+  a.hist_H = s->hist_H;
+  a.hist_Hfix = s->hist_Hfix;
+  a.hist_fixed = s->hist_fixed;
+  a.rec_ptr = s->d_rec;
+  a.hist_cap = s->p.history;
+  a.eligible = eligible;
+  a.fix_mode = s->p.fix_mode;
+  a.eps = 1.0e-10;
+  double R = s->p.L;
+  a.vol = std::pow(2 * R, 3.);
+  a.vol_over_N3 = std::pow(2 * R, 3.) / (double)N3;
+  EDB_TRY(launch_entropy_fix(a, batch, s->stream));
+  if (s->p.fix_mode == EDB_FIX_APPLIED && eligible) {
+    EDB_TRY(launch_fft3d(N, N, N, batch, ld, -1, 1.0, s->Fr, s->Ft, s->S2, s->stream));
+    EDB_TRY(launch_select_cols(N3, batch, ld, s->fixed, s->Ft, s->Fh, s->stream));
+  }
+  EDB_TRY(launch_bump(s->d_rec, s->stream));
+  return kOk;
+}
+
+// One pass of the driver's loop body (without the host-side t update).
+int step_body(edb_solver* s, int phase, int eligible) {
+  const double dt = s->p.dt;
+  double** Q = s->Q;
+  switch (s->p.scheme) {
+    case EDB_SCHEME_FEULER:  // BKW.h:220-222
+      EDB_TRY(qeval(s, s->Fh, nullptr, 0., Q[0]));
+      EDB_TRY(combine(s, kOpAxpy, s->Fh, s->Fh, nullptr, Q[0], nullptr, nullptr, nullptr, dt));
+      break;
+    case EDB_SCHEME_CFE: {  // BKW.h:281-283
+      double c = (1. - std::exp(-s->p.beta * dt)) / s->p.beta;
+      EDB_TRY(qeval(s, s->Fh, nullptr, 0., Q[0]));
+      EDB_TRY(combine(s, kOpAxpy, s->Fh, s->Fh, nullptr, Q[0], nullptr, nullptr, nullptr, c));
+      break;
+    }
+    case EDB_SCHEME_RK2:  // BKW.h:341-346
+      EDB_TRY(qeval(s, s->Fh, nullptr, 0., Q[0]));
+      EDB_TRY(qeval(s, s->Fh, Q[0], 0.5 * dt, Q[1]));
+      EDB_TRY(combine(s, kOpAxpy, s->Fh, s->Fh, nullptr, Q[1], nullptr, nullptr, nullptr, dt));
+      break;
+    case EDB_SCHEME_SSP3:  // BKW.h:416-424
+      EDB_TRY(qeval(s, s->Fh, nullptr, 0., Q[0]));
+      EDB_TRY(combine(s, kOpAxpy, s->S1, s->Fh, nullptr, Q[0], nullptr, nullptr, nullptr, dt));
+      EDB_TRY(qeval(s, s->S1, nullptr, 0., Q[1]));
+      EDB_TRY(combine(s, kOpSSP3b, s->S2, s->Fh, s->S1, Q[1], nullptr, nullptr, nullptr, 0.25 * dt));
+      EDB_TRY(qeval(s, s->S2, nullptr, 0., Q[2]));
+      EDB_TRY(combine(s, kOpSSP3c, s->Fh, s->Fh, s->S2, Q[2], nullptr, nullptr, nullptr, 2. / 3. * dt));
+      break;
+    case EDB_SCHEME_RK4:  // BKW.h:482-494
+      EDB_TRY(qeval(s, s->Fh, nullptr, 0., Q[0]));
+      EDB_TRY(qeval(s, s->Fh, Q[0], 0.5 * dt, Q[1]));
+      EDB_TRY(qeval(s, s->Fh, Q[1], 0.5 * dt, Q[2]));
+      EDB_TRY(qeval(s, s->Fh, Q[2], dt, Q[3]));
+      EDB_TRY(combine(s, kOpRK4, s->Fh, s->Fh, nullptr, Q[0], Q[1], Q[2], Q[3], dt / 6.));
+      break;
+    case EDB_SCHEME_DUFORT:
+      if (phase == 0) {  // start-up RK2 with step dt, BKW.h:551-556
+        EDB_TRY(qeval(s, s->Fh, nullptr, 0., Q[0]));
+        EDB_TRY(qeval(s, s->Fh, Q[0], 0.5 * dt, Q[1]));
+        EDB_TRY(combine(s, kOpAxpy, s->Fh, s->Fh, nullptr, Q[1], nullptr, nullptr, nullptr, dt));
+      } else {  // BKW.h:595-598, 630
+        const double b = s->p.beta;
+        EDB_TRY(qeval(s, s->Fh, nullptr, 0., Q[0]));
+        EDB_TRY(combine(s, kOpCopy, s->S1, s->Fh, nullptr, nullptr, nullptr, nullptr, nullptr, 0.));
+        EDB_TRY(combine(s, kOpDufort, s->Fh, s->S1, s->Fold, Q[0], nullptr, nullptr, nullptr,
+                        1. / (1. + b * dt), dt, b * dt));
+        EDB_TRY(combine(s, kOpCopy, s->Fold, s->S1, nullptr, nullptr, nullptr, nullptr, nullptr, 0.));
+      }
+      break;
+    default:
+      set_error("unknown scheme");
+      return kInvalid;
+  }
+  return post_step(s, eligible);
+}
+
+int run_one(edb_solver* s) {
+  const int scheme = s->p.scheme;
+  // fix eligibility of this step, per driver
+  bool fix_on = s->p.fix_mode != EDB_FIX_NONE;
+  if (s->p.fix_mode == EDB_FIX_DIAGNOSTIC && (scheme == EDB_SCHEME_RK2 || scheme == EDB_SCHEME_DUFORT))
+    fix_on = false;  // `if(0)` at BKW.h:358, 568, 610
+  int eligible = fix_on && (scheme == EDB_SCHEME_FEULER ? s->ref_step > 1 : s->ref_step > 0);
+  int phase = (scheme == EDB_SCHEME_DUFORT) ? s->dufort_phase : 0;
+  if (scheme == EDB_SCHEME_DUFORT && phase == 0) s->t += s->p.dt / 8;
+  else s->t += s->p.dt;
+
+  auto key = std::make_pair(phase, eligible);
+  if (s->use_graph) {
+    auto it = s->graphs.find(key);
+    if (it == s->graphs.end()) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      EDB_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      int st = step_body(s, phase, eligible);
+      hipError_t e = hipStreamEndCapture(s->stream, &graph);
+      if (st != kOk) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      EDB_HIP_TRY(e);
+      EDB_HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      it = s->graphs.emplace(key, exec).first;
+    }
+    EDB_HIP_TRY(hipGraphLaunch(it->second, s->stream));
+  } else {
+    EDB_TRY(step_body(s, phase, eligible));
+  }
+  s->ref_step++;
+  s->taken++;
+  if (scheme == EDB_SCHEME_DUFORT && phase == 0 && s->ref_step >= 8) {  // while(step < 8), BKW.h:587
+    s->dufort_phase = 1;
+    s->ref_step = 0;
+  }
+  return kOk;
+}
+
+int init_common(edb_solver* s, double T0) {
+  const int batch = s->p.batch, ld = s->ld;
+  s->t = T0;
+  s->taken = 0;
+  s->dufort_phase = 0;
+  s->ref_step = (s->p.scheme == EDB_SCHEME_FEULER) ? 1 : 0;
+  EDB_TRY(launch_fill(s->avg, ld, 1.0, s->stream));  // Averagef = 1.0, EntropyFix.h:20
+  EDB_TRY(launch_fill((double*)s->avg_flag, (ld + 1) / 2, 0., s->stream));
+  EDB_TRY(launch_fill((double*)s->d_rec, 1, 0., s->stream));
+  if (s->p.scheme == EDB_SCHEME_FEULER) {
+    // entropy0 = calEntropy() of the initial F, BKW.h:214; no fix, no record
+    EDB_TRY(launch_fft3d(s->p.N, s->p.N, s->p.N, batch, ld, +1, 1. / (double)s->g.N3, s->Fh, s->Fr,
+                         s->Ft, s->stream));
+    EntropyFixArgs a;
+    a.N3 = s->g.N3;
+    a.ld = ld;
+    a.F = (double2*)s->Fr;
+    a.W = s->S1;
+    a.H0 = s->H0;
+    a.avg = s->avg;
+    a.avg_flag = s->avg_flag;
+    a.eligible = 0;
+    a.fix_mode = 0;
+    a.vol = std::pow(2 * s->p.L, 3.);
+    a.vol_over_N3 = a.vol / (double)s->g.N3;
+    EDB_TRY(launch_entropy_fix(a, batch, s->stream));
+  } else {
+    EDB_TRY(launch_fill(s->H0, ld, 1., s->stream));  // entropy0 = 1., BKW.h:478
+  }
+  if (s->p.scheme == EDB_SCHEME_DUFORT)  // F_old = Fhat, BKW.h:539
+    EDB_TRY(combine(s, kOpCopy, s->Fold, s->Fh, nullptr, nullptr, nullptr, nullptr, nullptr, 0.));
+  EDB_HIP_TRY(hipStreamSynchronize(s->stream));
+  s->ready = true;
+  return kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edb_solver_create(const edb_solver_params* p, edb_qcalc* shared, edb_solver** out) {
+  EDB_REQUIRE(p && out, "solver_create: null argument");
+  *out = nullptr;
+  EDB_REQUIRE(p->batch >= 1 && p->batch <= 65536, "solver_create: batch must be in [1, 65536]");
+  EDB_REQUIRE(p->precision == EDB_PREC_FP64_PLAIN || p->precision == EDB_PREC_FP32_BATCHED,
+              "solver_create: unknown precision");
+  EDB_REQUIRE(p->scheme >= 0 && p->scheme <= 5, "solver_create: unknown scheme");
+  EDB_REQUIRE(p->fix_mode >= 0 && p->fix_mode <= 2, "solver_create: unknown fix mode");
+  EDB_REQUIRE(p->dt > 0. && std::isfinite(p->dt), "solver_create: dt must be positive");
+  EDB_REQUIRE((p->scheme != EDB_SCHEME_CFE && p->scheme != EDB_SCHEME_DUFORT) || p->beta > 0.,
+              "solver_create: CFE/DUFORT need beta > 0");
+  EDB_REQUIRE(p->history >= 0, "solver_create: history must be >= 0");
+  EDB_REQUIRE(p->N >= 3 && p->N <= kMaxNCalq, "solver_create: N must be in [3, 33]");
+  if (shared)
+    EDB_REQUIRE(shared->N == p->N && shared->alpha == p->alpha && shared->L == p->L &&
+                    shared->filter == (p->filter ? 1 : 0),
+                "solver_create: shared qcalc has different (N, alpha, L, filter)");
+  edb_solver* s = new edb_solver();
+  s->p = *p;
+  s->g = Geometry(p->N);
+  s->ld = pad64(p->batch);
+  int st = kOk;
+  if (shared) {
+    s->qc = shared;
+  } else {
+    st = edb_qcalc_create(p->N, p->alpha, p->L, &s->qc);
+    s->own_qc = st == kOk;
+    if (st == kOk && !p->filter) st = edb_qcalc_close_filter(s->qc);
+  }
+  if (st == kOk) st = batched(s) ? edb_qcalc_genB_batched(s->qc) : edb_qcalc_genB(s->qc);
+  if (st == kOk && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("solver_create: stream creation failed");
+    st = kHipError;
+  }
+  const size_t grid = (size_t)s->g.N3 * s->ld * 2, modes = (size_t)s->g.NQ * s->ld * 2;
+  if (st == kOk) st = dmalloc(&s->Fh, grid);
+  if (st == kOk) st = dmalloc(&s->Fr, grid);
+  if (st == kOk) st = dmalloc(&s->Ft, grid);
+  if (st == kOk) st = dmalloc(&s->S1, grid);
+  if (st == kOk) st = dmalloc(&s->S2, grid);
+  if (st == kOk && p->scheme == EDB_SCHEME_DUFORT) st = dmalloc(&s->Fold, grid);
+  for (int i = 0; i < 4 && st == kOk; ++i) st = dmalloc(&s->Q[i], modes);
+  if (st == kOk) st = dmalloc((double**)&s->in, modes);
+  if (st == kOk) st = dmalloc(&s->H0, (size_t)s->ld);
+  if (st == kOk) st = dmalloc(&s->avg, (size_t)s->ld);
+  if (st == kOk) st = dmalloc(&s->avg_flag, (size_t)s->ld + 2);
+  if (st == kOk) st = dmalloc(&s->fixed, (size_t)s->ld);
+  if (st == kOk) st = dmalloc((double**)&s->d_rec, 1);
+  if (st == kOk) st = dmalloc(&s->d_out, (size_t)s->ld * 5);
+  if (st == kOk && p->history > 0) {
+    st = dmalloc(&s->hist_H, (size_t)p->history * s->ld);
+    if (st == kOk) st = dmalloc(&s->hist_Hfix, (size_t)p->history * s->ld);
+    if (st == kOk) st = dmalloc(&s->hist_fixed, (size_t)p->history * s->ld);
+  }
+  if (st == kOk) st = launch_fill(s->S2, grid, 0., s->stream);
+  if (st == kOk) st = launch_fill((double*)s->fixed, (s->ld + 1) / 2, 0., s->stream);
+  if (st != kOk) {
+    std::string msg = g_err;
+    edb_solver_destroy(s);
+    set_error(msg);
+    return st;
+  }
+  *out = s;
+  return EDB_OK;
+}
+
+int edb_solver_set_initial_real(edb_solver* s, double T0, const double* F) {
+  EDB_REQUIRE(s && F, "set_initial_real: null argument");
+  const long N3 = s->g.N3;
+  double* dh = nullptr;
+  EDB_TRY(dmalloc(&dh, (size_t)N3 * s->p.batch));
+  int st = kOk;
+  if (hipMemcpy(dh, F, sizeof(double) * N3 * s->p.batch, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("set_initial_real: copy failed");
+    st = kHipError;
+  }
+  if (st == kOk) st = launch_real_to_complex_cols(N3, s->p.batch, s->ld, dh, s->Fr, s->stream);
+  if (st == kOk)
+    st = launch_fft3d(s->p.N, s->p.N, s->p.N, s->p.batch, s->ld, -1, 1.0, s->Fr, s->Fh, s->Ft, s->stream);
+  if (st == kOk && hipStreamSynchronize(s->stream) != hipSuccess) st = kHipError;
+  dfree(&dh);
+  if (st == kOk) st = init_common(s, T0);
+  return st;
+}
+
+int edb_solver_set_initial_spectral(edb_solver* s, double T0, const double* Fhat) {
+  EDB_REQUIRE(s && Fhat, "set_initial_spectral: null argument");
+  const long N3 = s->g.N3;
+  double* dh = nullptr;
+  EDB_TRY(dmalloc(&dh, (size_t)N3 * s->p.batch * 2));
+  int st = kOk;
+  if (hipMemcpy(dh, Fhat, sizeof(double) * 2 * N3 * s->p.batch, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("set_initial_spectral: copy failed");
+    st = kHipError;
+  }
+  if (st == kOk) st = launch_gather_cols(N3, s->p.batch, s->ld, dh, s->Fh, s->stream);
+  if (st == kOk && hipStreamSynchronize(s->stream) != hipSuccess) st = kHipError;
+  dfree(&dh);
+  if (st == kOk) st = init_common(s, T0);
+  return st;
+}
+
+int edb_solver_step(edb_solver* s, int nsteps) {
+  EDB_REQUIRE(s && nsteps >= 0, "solver_step: bad argument");
+  if (!s->ready) {
+    set_error("solver_step: initial data not set");
+    return EDB_ENOTREADY;
+  }
+  for (int i = 0; i < nsteps; ++i) EDB_TRY(run_one(s));
+  return EDB_OK;
+}
+
+int edb_solver_run(edb_solver* s, double TEnd, int* nsteps) {
+  EDB_REQUIRE(s, "solver_run: null handle");
+  if (!s->ready) {
+    set_error("solver_run: initial data not set");
+    return EDB_ENOTREADY;
+  }
+  int n = 0;
+  while (s->p.scheme == EDB_SCHEME_DUFORT && s->dufort_phase == 0) {  // while(step < 8), BKW.h:587
+    EDB_TRY(run_one(s));
+    ++n;
+  }
+  do {
+    EDB_TRY(run_one(s));
+    ++n;
+  } while (s->t < TEnd);
+  if (nsteps) *nsteps = n;
+  return EDB_OK;
+}
+
+int edb_solver_use_graph(edb_solver* s, int enable) {
+  EDB_REQUIRE(s, "null handle");
+  s->use_graph = enable != 0;
+  return EDB_OK;
+}
+
+int edb_solver_enable_qtiming(edb_solver* s, int enable) {
+  EDB_REQUIRE(s, "null handle");
+  s->qtiming = enable != 0;
+  s->ev_used = 0;
+  return EDB_OK;
+}
+
+int edb_solver_qkernel_time(edb_solver* s, double* ms_mean, long* launches) {
+  EDB_REQUIRE(s, "null handle");
+  EDB_HIP_TRY(hipStreamSynchronize(s->stream));
+  double tot = 0.;
+  for (size_t i = 0; i < s->ev_used; ++i) {
+    float el = 0.f;
+    EDB_HIP_TRY(hipEventElapsedTime(&el, s->ev[i].first, s->ev[i].second));
+    tot += el;
+  }
+  if (ms_mean) *ms_mean = s->ev_used ? tot / (double)s->ev_used : 0.;
+  if (launches) *launches = (long)s->ev_used;
+  s->ev_used = 0;
+  return EDB_OK;
+}
+
+int edb_solver_time(const edb_solver* s, double* t, int* steps) {
+  EDB_REQUIRE(s, "null handle");
+  if (t) *t = s->t;
+  if (steps) *steps = s->taken;
+  return EDB_OK;
+}
+
+int edb_solver_get_spectral(edb_solver* s, double* Fhat) {
+  EDB_REQUIRE(s && Fhat, "get_spectral: null argument");
+  const long N3 = s->g.N3;
+  double* dh = nullptr;
+  EDB_TRY(dmalloc(&dh, (size_t)N3 * s->p.batch * 2));
+  int st = launch_scatter_cols(N3, s->p.batch, s->ld, s->Fh, dh, s->stream);
+  if (st == kOk && hipStreamSynchronize(s->stream) != hipSuccess) st = kHipError;
+  if (st == kOk && hipMemcpy(Fhat, dh, sizeof(double) * 2 * N3 * s->p.batch, hipMemcpyDeviceToHost) !=
+                       hipSuccess)
+    st = kHipError;
+  if (st == kHipError) set_error("get_spectral: HIP failure");
+  dfree(&dh);
+  return st;
+}
+
+static int refresh_real(edb_solver* s) {
+  return launch_fft3d(s->p.N, s->p.N, s->p.N, s->p.batch, s->ld, +1, 1. / (double)s->g.N3, s->Fh,
+                      s->Ft, s->S2, s->stream);
+}
+
+int edb_solver_get_real(edb_solver* s, double* F) {
+  EDB_REQUIRE(s && F, "get_real: null argument");
+  const long N3 = s->g.N3;
+  double* dh = nullptr;
+  EDB_TRY(dmalloc(&dh, (size_t)N3 * s->p.batch));
+  int st = refresh_real(s);
+  if (st == kOk) st = launch_complex_to_real_cols(N3, s->p.batch, s->ld, s->Ft, dh, s->stream);
+  if (st == kOk && hipStreamSynchronize(s->stream) != hipSuccess) st = kHipError;
+  if (st == kOk &&
+      hipMemcpy(F, dh, sizeof(double) * N3 * s->p.batch, hipMemcpyDeviceToHost) != hipSuccess)
+    st = kHipError;
+  if (st == kHipError) set_error("get_real: HIP failure");
+  dfree(&dh);
+  return st;
+}
+
+int edb_solver_entropy_history(edb_solver* s, int steps, double* H, double* Hfix, int* fixed) {
+  EDB_REQUIRE(s, "null handle");
+  EDB_REQUIRE(steps >= 0 && steps <= s->p.history, "entropy_history: steps exceeds the history kept");
+  EDB_HIP_TRY(hipStreamSynchronize(s->stream));
+  const int b = s->p.batch, ld = s->ld;
+  std::vector<double> h((size_t)steps * ld), hf((size_t)steps * ld);
+  std::vector<int> fx((size_t)steps * ld);
+  if (steps > 0) {
+    EDB_HIP_TRY(hipMemcpy(h.data(), s->hist_H, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    EDB_HIP_TRY(hipMemcpy(hf.data(), s->hist_Hfix, sizeof(double) * hf.size(), hipMemcpyDeviceToHost));
+    EDB_HIP_TRY(hipMemcpy(fx.data(), s->hist_fixed, sizeof(int) * fx.size(), hipMemcpyDeviceToHost));
+  }
+  for (int r = 0; r < steps; ++r)
+    for (int i = 0; i < b; ++i) {
+      if (H) H[(size_t)r * b + i] = h[(size_t)r * ld + i];
+      if (Hfix) Hfix[(size_t)r * b + i] = hf[(size_t)r * ld + i];
+      if (fixed) fixed[(size_t)r * b + i] = fx[(size_t)r * ld + i];
+    }
+  return EDB_OK;
+}
+
+int edb_solver_moments(edb_solver* s, double* out) {
+  EDB_REQUIRE(s && out, "moments: null argument");
+  EDB_TRY(refresh_real(s));
+  EDB_TRY(launch_moments(s->p.N, s->p.batch, s->ld, s->p.L, s->Ft, s->d_out, s->stream));
+  EDB_HIP_TRY(hipStreamSynchronize(s->stream));
+  EDB_HIP_TRY(hipMemcpy(out, s->d_out, sizeof(double) * 5 * s->p.batch, hipMemcpyDeviceToHost));
+  return EDB_OK;
+}
+
+int edb_solver_bkw_error(edb_solver* s, double K, double* out) {
+  EDB_REQUIRE(s && out, "bkw_error: null argument");
+  EDB_REQUIRE(K > 0., "bkw_error: K must be positive");
+  EDB_TRY(refresh_real(s));
+  EDB_TRY(launch_bkw_error(s->p.N, s->p.batch, s->ld, s->p.L, K, s->Ft, s->d_out, s->stream));
+  EDB_HIP_TRY(hipStreamSynchronize(s->stream));
+  EDB_HIP_TRY(hipMemcpy(out, s->d_out, sizeof(double) * 2 * s->p.batch, hipMemcpyDeviceToHost));
+  return EDB_OK;
+}
+
+int edb_solver_destroy(edb_solver* s) {
+  if (!s) return EDB_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& e : s->ev) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  dfree(&s->Fh);
+  dfree(&s->Fr);
+  dfree(&s->Ft);
+  dfree(&s->S1);
+  dfree(&s->S2);
+  dfree(&s->Fold);
+  for (int i = 0; i < 4; ++i) dfree(&s->Q[i]);
+  dfree((double**)&s->in);
+  dfree(&s->H0);
+  dfree(&s->avg);
+  dfree(&s->avg_flag);
+  dfree(&s->fixed);
+  dfree(&s->hist_H);
+  dfree(&s->hist_Hfix);
+  dfree(&s->hist_fixed);
+  dfree((double**)&s->d_rec);
+  dfree(&s->d_out);
+  if (s->own_qc) edb_qcalc_destroy(s->qc);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return EDB_OK;
+}
+
+}  // extern "C"

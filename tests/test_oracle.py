@@ -1,0 +1,244 @@
+"""CPU checks that pin the oracle (oracle/edb_oracle.c) before it is trusted.
+
+Golden vectors come from the reference's own QCalculator.h / EntropyFix.h
+(tests/golden/make_golden.py); the rest from mathematics: quadrature of
+F_alpha, a brute-force (l, m)-order sum, numpy.fft, invariants, the BKW exact
+solution and the orders of accuracy of the explicit schemes.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import maxwellian_mixture, rand_spectrum
+
+L_REF = 6.62132
+
+
+@pytest.mark.parametrize("N", [3, 5, 7, 9])
+@pytest.mark.parametrize("alpha", [0, 1])
+@pytest.mark.parametrize("fil", [0, 1])
+def test_genB_bitwise_golden(golden, N, alpha, fil):
+    g = golden["genB"]
+    key = f"N{N}_a{alpha}_f{fil}"
+    B = oracle.genB(N, alpha, L_REF, fil).ravel()
+    assert hashlib.sha256(B.astype("<f8").tobytes()).digest() == bytes(g[key + "_sha256"])
+    np.testing.assert_array_equal(B[g[key + "_idx"]], g[key + "_val"])
+
+
+def _golden_q_cases(golden):
+    return sorted({k[:-2] for k in golden["calQ"] if k.endswith("_q")})
+
+
+def test_calQ_bitwise_golden(golden):
+    c = golden["calQ"]
+    for key in _golden_q_cases(golden):
+        if key.startswith("int_"):
+            N, alpha, fil = 5, 0, 1
+        else:
+            parts = key.split("_")
+            N, alpha, fil = int(parts[0][1:]), int(parts[1][1:]), int(parts[2][1:])
+        B = oracle.genB(N, alpha, L_REF, fil)
+        q = oracle.calQ(N, B, c[key + "_fin"], nthreads=4)
+        np.testing.assert_array_equal(q, c[key + "_q"], err_msg=key)
+
+
+def test_funF_quadrature():
+    from scipy import integrate
+    o = oracle.lib()
+    for alpha in (0, 1):
+        for xi, eta in [(0.0, 0.0), (1.4, 0.0), (0.0, 2.3), (2.0, 2.0), (1.4234, 2.8468),
+                        (5.1, 0.7), (9.3, 4.4)]:
+            def integrand(r):
+                s1 = np.sinc(xi * r / np.pi)
+                s2 = np.sinc(eta * r / np.pi)
+                return r ** (2 + alpha) * s1 * s2
+            ref, _ = integrate.quad(integrand, 0, 1, epsabs=1e-14, epsrel=1e-13, limit=200)
+            got = o.edbo_funF(alpha, xi, eta)
+            assert abs(got - ref) <= 2e-10 * max(1.0, abs(ref)), (alpha, xi, eta, got, ref)
+    assert o.edbo_funF(0, 0.0, 0.0) == 1.0 / 3.0 and o.edbo_funF(1, 0.0, 0.0) == 0.25
+    assert o.edbo_funF(2, 1.0, 1.0) == 0.0
+
+
+def test_filtering_values():
+    o = oracle.lib()
+    for N in (5, 9, 16, 24):
+        n = (N - 1) // 2
+        assert abs(o.edbo_filtering(N, 0) - 1.0) < 1e-15
+        assert abs(o.edbo_filtering(N, n + 1)) < 1e-15
+        vals = [o.edbo_filtering(N, b) for b in range(n + 2)]
+        assert all(vals[i] >= vals[i + 1] for i in range(n + 1))  # monotone taper
+        assert o.edbo_filtering(N, -2) == o.edbo_filtering(N, 2)
+
+
+def test_rval_closed_form():
+    o = oracle.lib()
+    lam = 2.0 / (3.0 + np.sqrt(2.0)) * np.pi
+    for N, a, L in [(9, 0, 6.62132), (24, 1, 8.0)]:
+        assert np.isclose(o.edbo_rval(N, a, L), 4 * np.pi * (2 * lam / np.pi * L / N) ** (3 + a),
+                          rtol=1e-15)
+
+
+@pytest.mark.parametrize("N", [3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("alpha,fil", [(0, 0), (1, 1)])
+def test_calQ_matches_direct_sum(N, alpha, fil):
+    """The fast (l, k-l) layout equals the direct (l, m) sum, odd and even N."""
+    f = rand_spectrum(N, 11 * N + alpha)
+    B = oracle.genB(N, alpha, 8.0, fil)
+    q = oracle.calQ(N, B, f)
+    qd = oracle.calQ_direct(N, f, alpha, 8.0, fil)
+    scale = np.abs(qd).max()
+    np.testing.assert_allclose(q, qd, rtol=0, atol=1e-13 * scale)
+
+
+def test_calQ_planted_mistake_is_caught():
+    """A transposed B (beta(m,l) instead of beta(l,m)) fails the direct sum."""
+    N = 5
+    f = rand_spectrum(N, 3)
+    B = oracle.genB(N, 0, 8.0, 0)
+    M = oracle.nmodes(N)
+    Bt = np.ascontiguousarray(B.T)
+    q_bad = oracle.calQ(N, Bt, f)
+    qd = oracle.calQ_direct(N, f, 0, 8.0, 0)
+    assert np.abs(q_bad - qd).max() > 1e-3 * np.abs(qd).max()
+    assert M == N
+
+
+@pytest.mark.parametrize("N", [5, 6, 9, 10])
+def test_invariants_mass_and_hermitian(N):
+    B = oracle.genB(N, 0, 8.0, 0)
+    f = rand_spectrum(N, 5, hermitian=True)
+    q = oracle.calQ(N, B, f)
+    assert q[0, 0, 0] == 0.0  # beta(l, -l) == 0 exactly: mass is conserved
+    # Hermitian symmetry Q_[-k] = conj(Q_k) on the Galerkin set
+    idx = (-np.arange(N)) % N
+    qm = q[np.ix_(idx, idx, idx)]
+    assert np.abs(qm - np.conj(q)).max() <= 1e-12 * np.abs(q).max()
+    if N % 2 == 0:
+        h = N // 2
+        assert np.all(q[h, :, :] == 0) and np.all(q[:, h, :] == 0) and np.all(q[:, :, h] == 0)
+
+
+def test_maxwellian_is_equilibrium():
+    """Q(Maxwellian) -> 0 spectrally fast as the grid resolves it (filter off)."""
+    L = 8.0
+    ratios = []
+    for N in (21, 25):
+        B = oracle.genB(N, 0, L, 0)
+        q = oracle.calQ(N, B, np.fft.fftn(oracle.grid_bkw(N, L, 1.0)))  # K = 1: Maxwellian
+        qb = oracle.calQ(N, B, np.fft.fftn(oracle.grid_bkw(N, L, 0.7)))
+        ratios.append(np.abs(q).max() / np.abs(qb).max())
+    assert ratios[1] < 1e-5 and ratios[1] < ratios[0] / 20, ratios
+
+
+@pytest.mark.parametrize("N,tol", [(17, 6e-2), (21, 8e-3), (24, 1.2e-2)])
+def test_bkw_consistency(N, tol):
+    """Q(DFT f_BKW) ~ d/dt DFT f_BKW with K(t) = 1 - exp(-t/6) (filter off)."""
+    L = 8.0
+    B = oracle.genB(N, 0, L, 0)
+    K = 0.64
+    fh = np.fft.fftn(oracle.grid_bkw(N, L, K))
+    q = oracle.calQ(N, B, fh)
+    dK = 1e-5
+    d = (np.fft.fftn(oracle.grid_bkw(N, L, K + dK)) - np.fft.fftn(oracle.grid_bkw(N, L, K - dK))) / (2 * dK)
+    d *= (1 - K) / 6.0
+    mask = np.ones((N, N, N), bool)
+    if N % 2 == 0:
+        h = N // 2
+        mask[h], mask[:, h], mask[:, :, h] = False, False, False
+    rel = np.linalg.norm((q - d)[mask]) / np.linalg.norm(d[mask])
+    assert rel < tol, rel
+
+
+@pytest.mark.parametrize("N", [4, 7, 8])
+def test_dft_matches_numpy(N):
+    rng = np.random.default_rng(N)
+    x = rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
+    np.testing.assert_allclose(oracle.fft(x), np.fft.fftn(x), rtol=0, atol=1e-12 * np.abs(x).sum() / N)
+    np.testing.assert_allclose(oracle.ifft(x), np.fft.ifftn(x), rtol=0, atol=1e-13 * np.abs(x).max())
+
+
+def test_entropyfix_golden(golden):
+    e = golden["efix"]
+    for N in (1000, 13824):
+        f1 = np.arange(N, dtype=np.float64)
+        f2 = np.empty(N)
+        f2[: N // 2] = np.floor((N - 1.0) / 2.0)
+        f2[N // 2:] = np.ceil((N - 1.0) / 2.0)
+        ef = oracle.EntropyFix(N)
+        h1 = ef.EntropyCal(f1)
+        h2 = ef.EntropyCal(f2)
+        fx = ef.FixSol(f1, h2)
+        h3 = ef.EntropyCal(fx)
+        np.testing.assert_array_equal([h1, h2, h3], e[f"seq{N}_H"])
+        assert hashlib.sha256(fx.astype("<f8").tobytes()).digest() == bytes(e[f"seq{N}_fixed_sha256"])
+    # the printed known answers of testEntropyFix (2 decimals)
+    assert f"{e['seq1000_H'][0]:.2f}" == "2700.92" and f"{e['seq1000_H'][2]:.2f}" == "2604.20"
+    assert f"{e['seq13824_H'][0]:.2f}" == "55527.86" and f"{e['seq13824_H'][2]:.2f}" == "54192.67"
+    for N in (64, 729, 4096):
+        ef = oracle.EntropyFix(N)
+        out = ef.FixSol(e[f"fix{N}_f"], float(e[f"fix{N}_aim"][0]))
+        np.testing.assert_array_equal(out, e[f"fix{N}_out"])
+        assert ef.e.average == e[f"fix{N}_avg_H"][0]
+
+
+def test_entropyfix_properties():
+    rng = np.random.default_rng(1)
+    f = rng.gamma(2.0, 0.1, 5000)
+    ef = oracle.EntropyFix(f.size)
+    H0 = ef.EntropyCal(f)
+    aim = H0 - 0.5 * (H0 - (f.mean() * np.log(f.mean()) - f.mean()))
+    fx = ef.FixSol(f, aim)
+    assert abs(fx.mean() - f.mean()) < 1e-14  # mass is preserved
+    assert ef.EntropyCal(fx) < aim + 1e-10
+
+
+def test_bkw_moments():
+    N, L = 32, 8.0
+    h = 2 * L / N
+    x = -L + h * np.arange(N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    for K in (0.5, 0.8, 1.0):
+        F = oracle.grid_bkw(N, L, K)
+        assert abs(F.sum() * h ** 3 - 1.0) < 1e-10
+        assert abs(((X ** 2 + Y ** 2 + Z ** 2) * F).sum() * h ** 3 - 3.0) < 1e-8
+        assert abs((X * F).sum()) * h ** 3 < 1e-12
+
+
+def _order(scheme, dts, N=7, T=0.4):
+    L = 8.0
+    B = oracle.genB(N, 0, L, 0)
+    fh = np.fft.fftn(maxwellian_mixture(N, L, 4, ncomp=2))
+    sols = [oracle.run(N, B, L, scheme, 0.0, T - 1e-9, dt, fh, max_steps=10)[0] for dt in dts]
+    e1 = np.abs(sols[0] - sols[2]).max()
+    e2 = np.abs(sols[1] - sols[2]).max()
+    return np.log2(e1 / e2)
+
+
+@pytest.mark.parametrize("scheme,order", [("FEuler", 1), ("RK2", 2), ("SSP3", 3), ("RK4", 4)])
+def test_scheme_orders(scheme, order):
+    """Each driver's stage table gives its order of accuracy (Richardson)."""
+    T = 0.8
+    dts = [T / 4, T / 8, T / 64]
+    p = _order(scheme, dts, T=T)
+    assert abs(p - order) < 0.35, (scheme, p)
+
+
+def test_run_step_count_and_entropy():
+    N, L = 11, 6.0
+    B = oracle.genB(N, 0, L, 0)
+    fh = np.fft.fftn(oracle.grid_bkw(N, L, 0.8))
+    f, t, H, Hf = oracle.run(N, B, L, "RK4", 0.0, 0.3, 0.1, fh, fix_mode=1)
+    assert np.all(np.isfinite(H))
+    # do { t += dt } while (t < TEnd) with t accumulated in double
+    tt, n = 0.0, 0
+    while True:
+        tt += 0.1
+        n += 1
+        if not tt < 0.3:
+            break
+    assert len(t) == n and t[-1] == tt
+    # the fix aims EntropyCal below the previous calEntropy; the functionals
+    # differ slightly, so a fix may barely move H (BKW.h:499-510)
+    assert np.all(Hf <= H + 1e-8 * np.abs(H))
