@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Benchmark of the CalQ path on one MI355X.
+
+Workload (one "step" = one RK4 step of the whole batch): 256 instances at
+N = 24 (23 Galerkin modes per dimension), Maxwell molecules (alpha = 0),
+L = 8, filter off, one shared FP32 collision kernel (paired layout), each
+initial f a seeded mixture of 2-4 isotropic Maxwellians normalised to mass 1,
+RK4 with dt = 0.01, entropy fix applied.  Each step is 4 batched
+Q-evaluations (4 x 256 Q-evals) + inverse FFT, entropy and fix.
+
+Prints one JSON line.  value = Q-evals per second over the timed steps
+(inputs resident in HBM).  roofline: the batched Q kernel's canonical work
+(10 M^6 flops per Q-eval) over its mean launch time, measured with HIP
+events on its own stream during the timed steps.  cpu_baseline: the FP64 CPU
+oracle (OpenMP over k) on a bounded sample of the same Q-evaluation.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Q-evals/sec at N=24 batched FP32 vs plain FP64 GPU; p50/p99 solve latency"
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak
+PEAK_HBM_GBS = 8000.0
+
+
+def mixture(N, L, seed):
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(2, 5))
+    w = rng.exponential(1.0, k)
+    w /= w.sum()
+    h = 2 * L / N
+    x = -L + h * np.arange(N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    f = np.zeros((N, N, N))
+    for i in range(k):
+        u = rng.uniform(-2, 2, 3)
+        T = rng.uniform(0.5, 1.5)
+        f += w[i] * np.exp(-((X - u[0]) ** 2 + (Y - u[1]) ** 2 + (Z - u[2]) ** 2) / (2 * T)) / (2 * np.pi * T) ** 1.5
+    return f / (f.sum() * h ** 3)
+
+
+def cpu_baseline(N, L, budget_s=12.0):
+    """FP64 CPU oracle Q-evaluations at N (OpenMP over k) for ~budget_s."""
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    B = oracle.genB(N, 0, L, False)
+    f = np.fft.fftn(mixture(N, L, 12345))
+    n, t0 = 0, time.perf_counter()
+    while True:
+        oracle.calQ(N, B, f, nthreads=threads)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 200:
+            break
+    return {"value": n / el, "unit": "Q-evals/s", "cores": threads, "kind": "port",
+            "sample": f"{n} FP64 Q-evaluations of one N={N} instance (oracle calQ, OpenMP over k, "
+                      f"{threads} threads) in {el:.1f} s; genB excluded"}
+
+
+def traffic_from_profiles():
+    p = os.path.join(ROOT, "profiles", "calq_batched_n24_b256_traffic.json")
+    if os.path.exists(p):
+        with open(p) as fh:
+            return json.load(fh).get("hbm_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--N", type=int, default=24)
+    ap.add_argument("--graph", action="store_true", help="replay each RK4 step from a HIP graph")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the collision path runs on a single MI355X here; "
+              "multi-GPU scaling is design-only (instance sharding).", file=sys.stderr)
+        sys.exit(2)
+
+    import entropydecreaseboltzmann_amd as edb
+
+    N, L, dt, batch = args.N, 8.0, 0.01, args.batch
+    M = edb.nmodes(N)
+    F0 = np.stack([mixture(N, L, 1000 + i) for i in range(batch)])
+    q = edb.QCalculator(N, 0, L)
+    q.closeFilter()
+    q.genB_batched()
+    s = edb.Solver(N, 0, L, False, batch, edb.PREC_FP32_BATCHED, "RK4", dt, 0.0, edb.FIX_APPLIED,
+                   hermitian=True, history=0, qcalc=q)
+    s.set_initial_real(F0)
+    s.use_graph(args.graph)
+    s.step(args.warmup)
+    edb.load().edb_sync()
+    s.enable_qtiming(not args.graph)
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    edb.load().edb_sync()
+    el = time.perf_counter() - t0
+    qms, nlaunch = s.qkernel_time() if not args.graph else (0.0, 0)
+    if args.graph:  # kernel timing from a short instrumented eager pass
+        s.use_graph(False)
+        s.enable_qtiming(True)
+        s.step(2)
+        qms, nlaunch = s.qkernel_time()
+    qevals = 4 * batch * args.steps
+    value = qevals / el
+    flops_launch = 10.0 * M ** 6 * batch
+    achieved = flops_launch / (qms * 1e-3) / 1e12
+    traffic = traffic_from_profiles()
+    mom = s.moments()
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Q-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batched RK4 step, {batch} instances, N={N} ({M}^3 modes), alpha=0, "
+                               f"L=8, FP32 paired beta, entropy fix applied, dt={dt}",
+                   "N": N, "batch": batch, "alpha": 0, "scheme": "RK4", "graph": bool(args.graph)},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": traffic, "kernel": "calq_batched_kernel",
+                     "kernel_ms_mean": round(qms, 4), "launches": nlaunch,
+                     "flops_per_launch": flops_launch},
+        "mass_drift_max": float(np.abs(mom[:, 0] - 1.0).max()),
+    }
+    if not args.no_plain:
+        qp = edb.QCalculator(N, 0, L)
+        qp.closeFilter()
+        qp.genB()
+        pms = qp.time_plain(reps=5)
+        out["plain_fp64"] = {"qevals_per_s": round(1e3 / pms, 1), "kernel_ms_mean": round(pms, 4),
+                             "beta_gbs": round(M ** 6 * 8 / (pms * 1e-3) / 1e9, 1),
+                             "hbm_frac": round(M ** 6 * 8 / (pms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+        out["speedup_vs_plain_fp64"] = round(value / out["plain_fp64"]["qevals_per_s"], 2)
+        qp.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(N, L)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

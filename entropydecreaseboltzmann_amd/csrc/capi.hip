@@ -210,6 +210,7 @@ int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out) {
   q->ntile = (int)((q->g.NQ + 63) / 64);
   // enough l-slabs to give every SIMD several waves of the HBM stream
   q->nslab = std::max(1, std::min(64, (cu_count() * 16 + q->ntile - 1) / q->ntile));
+  q->nslab = (int)std::min<long>(q->nslab, q->g.NQ);
   hipError_t e = hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -772,8 +773,26 @@ int run_one(edb_solver* s) {
   return kOk;
 }
 
+// Everything a step allocates lazily, allocated now, so that a step can be
+// captured into a HIP graph (no allocation during capture).
+int prepare_buffers(edb_solver* s) {
+  EDB_TRY(fft_prepare(s->p.N));
+  if (batched(s)) {
+    const int groups = (s->p.batch + 63) / 64;
+    PrefixEntry* pe = nullptr;
+    EDB_TRY(get_prefix(s->qc, groups, s->p.hermitian ? 1 : 0, &pe));
+    int nwaves = s->qc->nwaves > 0 ? s->qc->nwaves : cu_count() * 8;
+    if (nwaves > pe->total) nwaves = (int)pe->total;
+    EDB_TRY(ensure_slots(s->qc, nwaves));
+  } else {
+    EDB_TRY(ensure_plain_scratch(s->qc));
+  }
+  return kOk;
+}
+
 int init_common(edb_solver* s, double T0) {
   const int batch = s->p.batch, ld = s->ld;
+  EDB_TRY(prepare_buffers(s));
   s->t = T0;
   s->taken = 0;
   s->dufort_phase = 0;

@@ -107,7 +107,7 @@ int twiddles(int NA, int sign, const double** out) {
   auto key = std::make_tuple(dev, NA, sign);
   auto itr = g_tw.find(key);
   if (itr != g_tw.end()) {
-    *out = itr->second;
+    if (out) *out = itr->second;
     return kOk;
   }
   std::vector<double> h(2 * NA);
@@ -120,7 +120,7 @@ int twiddles(int NA, int sign, const double** out) {
   EDB_HIP_TRY(hipMalloc(&d, sizeof(double) * 2 * NA));
   EDB_HIP_TRY(hipMemcpy(d, h.data(), sizeof(double) * 2 * NA, hipMemcpyHostToDevice));
   g_tw[key] = d;
-  *out = d;
+  if (out) *out = d;
   return kOk;
 }
 
@@ -145,6 +145,14 @@ int run_pass(int NA, const double2* in, double2* out, int ld, int groups, long n
 }
 
 }  // namespace
+
+// Create the twiddle tables of an N^3 grid ahead of time (a stream capture
+// must not allocate).
+int fft_prepare(int N) {
+  EDB_TRY(twiddles(N, -1, nullptr));
+  EDB_TRY(twiddles(N, +1, nullptr));
+  return kOk;
+}
 
 // sign -1: forward; +1: backward.  scale multiplies the final pass.  d_tmp
 // (same size as d_out) is needed only when an axis length has no compiled

@@ -72,6 +72,11 @@ __device__ inline double beta_lk(const BetaArgs& a, int l1c, int l2c, int l3c, i
       !wrap_freq(a.N, k3c - l3c, &m3))
     return 0.;
   long A = (long)(l1 + m1) * (l1 + m1) + (long)(l2 + m2) * (l2 + m2) + (long)(l3 + m3) * (l3 + m3);
+  // m = -l: F(0, lambda|2l|) sigma_l^2 rval - F(lambda|2l|, 0) sigma_l^2 rval, which the
+  // reference evaluates to exactly 0 (same formula and arguments, QCalculator.h:84,87);
+  // Bmm comes from host libm here, so the zero is kept exact explicitly.  It is what
+  // makes Q_0 == 0, i.e. exact mass conservation.
+  if (A == 0) return 0.;
   long Bq = (long)(l1 - m1) * (l1 - m1) + (long)(l2 - m2) * (l2 - m2) + (long)(l3 - m3) * (l3 - m3);
   double sigl = 1., sigm = 1.;
   if (a.filter) {
