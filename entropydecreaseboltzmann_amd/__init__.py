@@ -113,6 +113,15 @@ class QCalculator:
         check(load().edb_qcalc_time_plain(self._h, int(reps), ctypes.byref(ms)), "time_plain")
         return ms.value
 
+    def tune(self, batched_waves=0, plain_slabs=0):
+        check(load().edb_qcalc_tune(self._h, int(batched_waves), int(plain_slabs)), "tune")
+
+    def autotune(self, batch=256):
+        bw, ps = ctypes.c_int(), ctypes.c_int()
+        check(load().edb_qcalc_autotune(self._h, int(batch), ctypes.byref(bw), ctypes.byref(ps)),
+              "autotune")
+        return {"batched_waves": bw.value, "plain_slabs": ps.value}
+
     @property
     def handle(self):
         return self._h

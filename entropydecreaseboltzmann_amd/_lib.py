@@ -64,6 +64,8 @@ _SIGS = {
     "edb_qcalc_info": (ctypes.c_int, [_vp, _ip, _ip, _lp, _lp]),
     "edb_qcalc_time_batched": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "edb_qcalc_time_plain": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    "edb_qcalc_tune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "edb_qcalc_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
     "edb_qcalc_destroy": (ctypes.c_int, [_vp]),
     "edb_fft3d_fft": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "edb_fft3d_ifft": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),

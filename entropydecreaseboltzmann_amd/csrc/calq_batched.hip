@@ -15,11 +15,14 @@
 // it is read with scalar loads and used as an SGPR operand -- every beta
 // entry is fetched once per 64 instances.  A wave owns one k-row
 // (k1,k2) x all k3 (or half of them for the largest N) of 64 instances and
-// walks that row's schedule of l-rows (l1,l2); for each it holds the M
-// values f[m1,m2,:] in registers and streams f[l1,l2,:], and the M x M table
-// beta_s[l3][k3] drives a fully unrolled product loop whose partner index
-// m3 = [k3-l3] is resolved at compile time.  Each l-row's FP32 partial sums
-// are folded into FP64 accumulators (blocked compensation).
+// walks that row's schedule of l-rows (l1,l2); for each it loads the rows
+// f[l1,l2,:] and f[m1,m2,:] into registers up front (in-order vector loads
+// consumed progressively), and the M x M table beta_s[l3][k3] drives a fully
+// unrolled product loop whose partner index m3 = [k3-l3] is resolved at
+// compile time.  beta rows are prefetched one row ahead into SGPRs behind an
+// explicit wait (scalar loads return out of order, so a later wait would also
+// cover the prefetch).  Each l-row's FP32 partial sums are folded into FP64
+// accumulators (blocked compensation).
 //
 // Load balance.  The kernel is persistent: nwaves waves split the flattened
 // sequence of (k-row, k3-half, instance group, l-row) steps into equal
@@ -74,31 +77,47 @@ __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__
                                          const float2* __restrict__ fcr, int Bp, float (&pr)[KPT],
                                          float (&pi)[KPT]) {
   constexpr int n = (N - 1) / 2, M = 2 * n + 1;
-  float2 c[M];
+  constexpr int K0 = H * KPT, K1 = (H + 1) * KPT < M ? (H + 1) * KPT : M;
+  constexpr int NK = K1 - K0;
+  float2 c[M], a[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) c[m] = fcr[(long)m * Bp];
+#pragma unroll
+  for (int m = 0; m < M; ++m) a[m] = fa[(long)m * Bp];
 #pragma unroll
   for (int j = 0; j < KPT; ++j) {
     pr[j] = 0.f;
     pi[j] = 0.f;
   }
+  float bcur[NK], bnext[NK];
+#pragma unroll
+  for (int j = 0; j < NK; ++j) bcur[j] = bb[K0 + j];
 #pragma unroll
   for (int l3 = 0; l3 < M; ++l3) {
-    const float2 a = fa[(long)l3 * Bp];
+    // Scalar loads complete out of order, so the only wait is lgkmcnt(0):
+    // wait for row l3 here, then issue row l3+1 behind a scheduling barrier
+    // so that the wait the compiler inserts for row l3 precedes the prefetch.
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_sched_barrier(0);
+    if (l3 + 1 < M) {
 #pragma unroll
-    for (int j = 0; j < KPT; ++j) {
-      const int k3 = H * KPT + j;
-      if (k3 < M) {
-        const int m = cwrapc(N, k3 - l3);
-        if (m >= 0) {
-          const float b = bb[l3 * M + k3];
-          const float xr = a.x * c[m].x - a.y * c[m].y;
-          const float xi = a.x * c[m].y + a.y * c[m].x;
-          pr[j] += b * xr;
-          pi[j] += b * xi;
-        }
+      for (int j = 0; j < NK; ++j) bnext[j] = bb[(l3 + 1) * M + K0 + j];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+      const int k3 = K0 + j;
+      const int m = cwrapc(N, k3 - l3);
+      if (m >= 0) {
+        const float b = bcur[j];
+        const float xr = a[l3].x * c[m].x - a[l3].y * c[m].y;
+        const float xi = a[l3].x * c[m].y + a[l3].y * c[m].x;
+        pr[j] += b * xr;
+        pi[j] += b * xi;
       }
     }
+#pragma unroll
+    for (int j = 0; j < NK; ++j) bcur[j] = bnext[j];
   }
 }
 
@@ -251,7 +270,7 @@ BatchedFn lookup(int N, int ks) {
 }  // namespace
 
 int batched_ksplit(int N) { return ksplit_of(N); }
-bool batched_supported(int N) { return lookup(N, 1) != nullptr; }
+bool batched_supported(int N) { return lookup(N, ksplit_of(N)) != nullptr; }
 
 // Host-side schedule: rows ordered Hermitian half first; per row the l-rows
 // (l1,l2) whose partner (m1,m2) = ([k1-l1], [k2-l2]) is a mode and which is

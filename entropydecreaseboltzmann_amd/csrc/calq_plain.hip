@@ -5,9 +5,11 @@
 // one of the M^6 FP64 beta entries is streamed once from HBM, so the kernel
 // is HBM-bound (8 bytes and 10 flops per term).
 //
-// Layout.  beta is pre-tiled as [tile][l][64]: a wave owns 64 consecutive
-// centred k (one per lane) and walks a slab of l, so each l step is one
-// coalesced 512-byte load.  f_l is wave-uniform (scalar load).  f_[k-l] is
+// Layout.  beta is the reference's row-major [l][k] (rows padded to a multiple
+// of 64): a wave owns 64 consecutive centred k (one per lane) and walks a slab
+// of l, so each l step is one coalesced 512-byte load, and the waves of a slab
+// sweep one contiguous front of memory together (DRAM row locality).  beta is
+// read once, with non-temporal loads, so it does not evict the gather table.  f_l is wave-uniform (scalar load).  f_[k-l] is
 // gathered from a duplicated table D over differences d = k-l in
 // [-(M-1), M-1]^3 (the role of QCalculator::transf, QCalculator.h:108-123),
 // D[d] = f_[d]_N or 0 on the Nyquist residue, so the lane's address is its
@@ -48,7 +50,8 @@ __global__ __launch_bounds__(64) void calq_plain64_kernel(int M, long NQ, const 
   int k1 = (int)(kcc / ((long)M * M)), k2 = (int)((kcc / M) % M), k3 = (int)(kcc % M);
   long dbase = ((long)(k1 + M - 1) * W + (k2 + M - 1)) * W + (k3 + M - 1);
   long l0 = NQ * slab / nslab, l1e = NQ * (slab + 1) / nslab;
-  const double* bt = B + ((long)tile * NQ) * 64 + lane;
+  const long kp = (long)gridDim.x * 64;  // padded row length
+  const double* bt = B + (long)tile * 64 + lane;
   double qr = 0., qi = 0.;
   int a1 = (int)(l0 / ((long)M * M)), a2 = (int)((l0 / M) % M), a3 = (int)(l0 % M);
   long l = l0;
@@ -58,7 +61,7 @@ __global__ __launch_bounds__(64) void calq_plain64_kernel(int M, long NQ, const 
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
       long lin = ((long)a1 * W + a2) * W + a3;
-      b[u] = bt[(l + u) * 64];
+      b[u] = __builtin_nontemporal_load(bt + (l + u) * kp);
       c[u] = dup[dbase - lin];
       a[u] = fc[(l + u) * stride];
       if (++a3 == M) { a3 = 0; if (++a2 == M) { a2 = 0; ++a1; } }
@@ -73,7 +76,7 @@ __global__ __launch_bounds__(64) void calq_plain64_kernel(int M, long NQ, const 
   }
   for (; l < l1e; ++l) {
     long lin = ((long)a1 * W + a2) * W + a3;
-    double b = bt[l * 64];
+    double b = __builtin_nontemporal_load(bt + l * kp);
     double2 c = dup[dbase - lin];
     double2 a = fc[l * stride];
     double pr = a.x * c.x - a.y * c.y;

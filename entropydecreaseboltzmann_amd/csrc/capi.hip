@@ -209,7 +209,7 @@ int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out) {
   q->g = Geometry(N);
   q->ntile = (int)((q->g.NQ + 63) / 64);
   // enough l-slabs to give every SIMD several waves of the HBM stream
-  q->nslab = std::max(1, std::min(64, (cu_count() * 16 + q->ntile - 1) / q->ntile));
+  q->nslab = std::max(1, std::min(64, (cu_count() * 12 + q->ntile - 1) / q->ntile));
   q->nslab = (int)std::min<long>(q->nslab, q->g.NQ);
   hipError_t e = hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -444,6 +444,60 @@ int edb_qcalc_time_plain(edb_qcalc* q, int reps, double* ms) {
   dfree(&fc);
   dfree(&dq);
   return st;
+}
+
+int edb_qcalc_tune(edb_qcalc* q, int batched_waves, int plain_slabs) {
+  EDB_REQUIRE(q, "null handle");
+  EDB_REQUIRE(batched_waves >= 0 && batched_waves <= (1 << 20), "tune: batched_waves out of range");
+  EDB_REQUIRE(plain_slabs >= 0 && plain_slabs <= 4096, "tune: plain_slabs out of range");
+  std::lock_guard<std::mutex> lk(q->mu);
+  q->nwaves = batched_waves;
+  if (plain_slabs > 0) {
+    int ns = (int)std::min<long>(plain_slabs, q->g.NQ);
+    if (ns != q->nslab) {
+      dfree(&q->d_part);
+      dfree(&q->d_dup);
+      q->nslab = ns;
+    }
+  }
+  return EDB_OK;
+}
+
+int edb_qcalc_autotune(edb_qcalc* q, int batch, int* batched_waves, int* plain_slabs) {
+  EDB_REQUIRE(q, "null handle");
+  const int cu = cu_count();
+  if (q->d_pair && batch >= 1) {
+    double best = 1e30;
+    int bw = 0;
+    for (int per_cu : {4, 6, 8, 12, 16}) {
+      q->nwaves = cu * per_cu;
+      double ms = 0.;
+      EDB_TRY(edb_qcalc_time_batched(q, batch, 1, 3, &ms));
+      if (ms < best) {
+        best = ms;
+        bw = q->nwaves;
+      }
+    }
+    q->nwaves = bw;
+  }
+  if (q->d_plain) {
+    double best = 1e30;
+    int bs = q->nslab;
+    for (int waves_per_cu : {8, 16, 24, 32, 48, 64}) {
+      int ns = std::max(1, std::min(4096, (cu * waves_per_cu + q->ntile - 1) / q->ntile));
+      EDB_TRY(edb_qcalc_tune(q, q->nwaves, ns));
+      double ms = 0.;
+      EDB_TRY(edb_qcalc_time_plain(q, 3, &ms));
+      if (ms < best) {
+        best = ms;
+        bs = q->nslab;
+      }
+    }
+    EDB_TRY(edb_qcalc_tune(q, q->nwaves, bs));
+  }
+  if (batched_waves) *batched_waves = q->nwaves;
+  if (plain_slabs) *plain_slabs = q->nslab;
+  return EDB_OK;
 }
 
 int edb_qcalc_destroy(edb_qcalc* q) {

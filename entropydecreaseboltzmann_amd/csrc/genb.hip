@@ -96,16 +96,15 @@ __global__ void genb_ref64_kernel(BetaArgs a, double* __restrict__ B, long NQ) {
                    (int)((kc / M) % M), (int)(kc % M));
 }
 
-// Plain-kernel layout: [tile][l][64], k = tile*64 + lane (zero padded).
+// Plain-kernel layout: the reference's row-major [l][k] with each row padded
+// to ntile*64 entries (zeros past M^3), k = tile*64 + lane.
 __global__ void genb_plain64_kernel(BetaArgs a, double* __restrict__ B, long NQ, int ntile) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long total = (long)ntile * NQ * 64;
+  const long kp = (long)ntile * 64;
+  long total = kp * NQ;
   if (idx >= total) return;
-  int kk = (int)(idx & 63);
-  long rest = idx >> 6;
-  long lc = rest % NQ;
-  long t = rest / NQ;
-  long kc = t * 64 + kk;
+  long lc = idx / kp;
+  long kc = idx % kp;
   int M = a.M;
   double v = 0.;
   if (kc < NQ)

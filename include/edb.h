@@ -120,6 +120,15 @@ int edb_qcalc_time_batched(edb_qcalc* q, int batch, int hermitian, int reps, dou
 /* Same for the plain FP64 kernel (one instance). */
 int edb_qcalc_time_plain(edb_qcalc* q, int reps, double* ms);
 
+/* Launch-shape knobs: persistent waves of the batched kernel and l-slabs of
+ * the plain kernel (0 = built-in default).  No reference counterpart. */
+int edb_qcalc_tune(edb_qcalc* q, int batched_waves, int plain_slabs);
+
+/* Autotuner: times the candidate launch shapes of the kernels built so far
+ * (batched at `batch` instances, Hermitian half) with HIP events and keeps
+ * the fastest; returns them.  No reference counterpart. */
+int edb_qcalc_autotune(edb_qcalc* q, int batch, int* batched_waves, int* plain_slabs);
+
 int edb_qcalc_destroy(edb_qcalc* q);
 
 /* ------------------------------------------------------ fft3d (L2) ----- */

@@ -64,7 +64,7 @@ def test_plain_calQ_even_N(edb, N):
     assert rel(q.calQ(f), oracle.calQ(N, B, f)) < 1e-12
 
 
-@pytest.mark.parametrize("N,batch", [(5, 1), (8, 3), (9, 70), (16, 2)])
+@pytest.mark.parametrize("N,batch", [(5, 1), (8, 3), (9, 70), (16, 2), (25, 1)])
 def test_batched_calQ_general_input(edb, N, batch):
     fs = np.stack([rand_spectrum(N, 7 * N + i) for i in range(batch)])
     q = edb.QCalculator(N, 1, 8.0)
@@ -77,7 +77,7 @@ def test_batched_calQ_general_input(edb, N, batch):
         assert np.all(np.abs(got[i] - ref) <= 3e-6 * S + 1e-300), (N, i)
 
 
-@pytest.mark.parametrize("N,batch", [(9, 4), (16, 65), (17, 2), (24, 2)])
+@pytest.mark.parametrize("N,batch", [(9, 4), (16, 65), (17, 2), (24, 2), (32, 1)])
 def test_batched_calQ_hermitian(edb, N, batch):
     L = 8.0
     fs = np.stack([np.fft.fftn(maxwellian_mixture(N, L, 40 + i)) for i in range(batch)])
