@@ -22,7 +22,7 @@ from . import _lib
 from ._lib import (FIX_APPLIED, FIX_DIAGNOSTIC, FIX_NONE, PREC_FP32_BATCHED,
                    PREC_FP64_PLAIN, SCHEMES, EdbError, check, load)
 
-__all__ = ["QCalculator", "fft3d", "EntropyFix", "Solver", "EdbError", "load",
+__all__ = ["QCalculator", "fft3d", "EntropyFix", "Solver", "Server", "EdbError", "load",
            "device_count", "nmodes", "SCHEMES", "FIX_NONE", "FIX_DIAGNOSTIC",
            "FIX_APPLIED", "PREC_FP64_PLAIN", "PREC_FP32_BATCHED"]
 
@@ -276,6 +276,59 @@ class Solver:
     def close(self):
         if self._h:
             load().edb_solver_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous batching of independent RK4 solves (include/edb.h, edb_server_*)."""
+
+    def __init__(self, L=8.0, filter=False, dt=0.01, fix_mode=FIX_APPLIED, max_slots=0, max_chunk=0,
+                 hbm_budget=0):
+        p = _lib.ServerParams()
+        p.L, p.filter, p.dt, p.fix_mode = float(L), int(bool(filter)), float(dt), int(fix_mode)
+        p.max_slots, p.max_chunk, p.hbm_budget = int(max_slots), int(max_chunk), int(hbm_budget)
+        self.p = p
+        self._h = ctypes.c_void_p()
+        check(load().edb_server_create(ctypes.byref(p), ctypes.byref(self._h)), "edb_server_create")
+
+    def submit(self, rid, N, alpha, t_end, F0):
+        """Returns the status (0 ok, EDB_ENOMEM never fits, EDB_EBUSY not now)."""
+        F = np.ascontiguousarray(F0, dtype=np.float64).ravel()
+        if F.size != N ** 3:
+            raise ValueError("F0 must have N^3 values")
+        st = load().edb_server_submit(self._h, int(rid), int(N), int(alpha), float(t_end), _ptr(F))
+        if st not in (_lib.EDB_OK, _lib.EDB_ENOMEM, _lib.EDB_EBUSY):
+            check(st, "server.submit")
+        return st
+
+    def poll(self):
+        n = ctypes.c_int()
+        check(load().edb_server_poll(self._h, ctypes.byref(n)), "server.poll")
+        return n.value
+
+    def fetch(self, max_N=33):
+        rid, N, steps = ctypes.c_long(), ctypes.c_int(), ctypes.c_int()
+        t = ctypes.c_double()
+        buf = np.empty(max_N ** 3)
+        check(load().edb_server_fetch(self._h, ctypes.byref(rid), ctypes.byref(N), ctypes.byref(steps),
+                                      ctypes.byref(t), _ptr(buf), buf.size), "server.fetch")
+        n = N.value
+        return rid.value, n, steps.value, t.value, buf[: n ** 3].reshape(n, n, n).copy()
+
+    def stats(self):
+        v = [ctypes.c_long() for _ in range(5)]
+        check(load().edb_server_stats(self._h, *[ctypes.byref(x) for x in v]), "server.stats")
+        return dict(zip(["submitted", "finished", "active", "waiting", "hbm_used"], [x.value for x in v]))
+
+    def close(self):
+        if self._h:
+            load().edb_server_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

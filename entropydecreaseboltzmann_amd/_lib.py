@@ -46,6 +46,12 @@ class SolverParams(ctypes.Structure):
                 ("history", ctypes.c_int)]
 
 
+class ServerParams(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_double), ("filter", ctypes.c_int), ("dt", ctypes.c_double),
+                ("fix_mode", ctypes.c_int), ("max_slots", ctypes.c_int), ("max_chunk", ctypes.c_int),
+                ("hbm_budget", ctypes.c_long)]
+
+
 _SIGS = {
     "edb_last_error": (ctypes.c_char_p, []),
     "edb_version": (ctypes.c_int, [_ip, _ip]),
@@ -89,6 +95,12 @@ _SIGS = {
     "edb_solver_moments": (ctypes.c_int, [_vp, _dp]),
     "edb_solver_bkw_error": (ctypes.c_int, [_vp, ctypes.c_double, _dp]),
     "edb_solver_destroy": (ctypes.c_int, [_vp]),
+    "edb_server_create": (ctypes.c_int, [ctypes.POINTER(ServerParams), ctypes.POINTER(_vp)]),
+    "edb_server_submit": (ctypes.c_int, [_vp, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp]),
+    "edb_server_poll": (ctypes.c_int, [_vp, _ip]),
+    "edb_server_fetch": (ctypes.c_int, [_vp, _lp, _ip, _ip, _dp, _dp, ctypes.c_long]),
+    "edb_server_stats": (ctypes.c_int, [_vp, _lp, _lp, _lp, _lp, _lp]),
+    "edb_server_destroy": (ctypes.c_int, [_vp]),
 }
 
 _lib = None

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/edb.h"
+#include "capi_internal.hpp"
 #include "edb_internal.hpp"
 #include "ops.hpp"
 
@@ -17,31 +18,6 @@ namespace edb {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 const char* last_error() { return g_err.c_str(); }
-
-namespace {
-
-template <typename T>
-int dmalloc(T** p, size_t count) {
-  *p = nullptr;
-  if (count == 0) return kOk;
-  hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    set_error(std::string("hipMalloc(") + std::to_string(sizeof(T) * count) + " bytes): " +
-              hipGetErrorString(e));
-    (void)hipGetLastError();
-    return e == hipErrorOutOfMemory ? kNoMemory : kHipError;
-  }
-  return kOk;
-}
-
-template <typename T>
-void dfree(T** p) {
-  if (*p) (void)hipFree((void*)*p);
-  *p = nullptr;
-}
-
-int pad64(int b) { return ((b + 63) / 64) * 64; }
 
 int cu_count() {
   static int cached = 0;
@@ -54,45 +30,12 @@ int cu_count() {
   return n;
 }
 
-struct PrefixEntry {
-  long* d_prefix = nullptr;
-  int nitems = 0;
-  long total = 0;
-};
-
-}  // namespace
 }  // namespace edb
 
 using namespace edb;
 
-struct edb_qcalc {
-  int N = 0, alpha = 0, filter = 1;
-  double L = 0.;
-  Geometry g;
-  hipStream_t stream = nullptr;
-  BetaTables tables;
-  bool tables_ready = false;
-  // plain FP64
-  double* d_plain = nullptr;
-  int ntile = 0;
-  int nslab = 16;
-  double* d_dup = nullptr;
-  double* d_part = nullptr;
-  // paired FP32
-  float* d_pair = nullptr;
-  int4* d_rowinfo = nullptr;
-  uint32_t* d_sched = nullptr;
-  std::vector<int4> rowinfo;
-  int nrows_half = 0;
-  long nblocks = 0;
-  std::map<std::pair<int, int>, PrefixEntry> prefixes;
-  double* d_slots = nullptr;
-  size_t slots_count = 0;
-  int nwaves = 0;
-  std::mutex mu;
-};
 
-namespace {
+namespace edb {
 
 int ensure_tables(edb_qcalc* q) {
   if (q->tables_ready) return kOk;
@@ -133,7 +76,7 @@ int ensure_slots(edb_qcalc* q, int nwaves) {
 
 // Batched Q on device arrays already in the kernel layout.
 int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, const float* d_fin,
-                   hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+                   hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   EDB_REQUIRE(q->d_pair, "calQ_batched: genB_batched has not run");
   EDB_REQUIRE(batch >= 1 && ld % 64 == 0 && batch <= ld, "calQ_batched: bad batch / ld");
   const int groups = (batch + 63) / 64;
@@ -158,7 +101,7 @@ int ensure_plain_scratch(edb_qcalc* q) {
 // Plain FP64 Q of one column: d_fc FP64 centred [M^3] with stride fc_stride
 // (complex elements), result centred into d_q with stride q_stride.
 int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, int q_stride,
-                 hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+                 hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   EDB_REQUIRE(q->d_plain, "calQ: genB has not run");
   EDB_TRY(ensure_plain_scratch(q));
   EDB_TRY(launch_build_dup(q->N, d_fc, fc_stride, q->d_dup, s));
@@ -169,7 +112,7 @@ int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, i
   return kOk;
 }
 
-}  // namespace
+}  // namespace edb
 
 extern "C" {
 

@@ -1,0 +1,88 @@
+// capi_internal.hpp -- state of a QCalculator handle and the host helpers
+// shared by the C ABI (capi.hip) and the request server (server.hip).
+#pragma once
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "edb_internal.hpp"
+#include "ops.hpp"
+
+namespace edb {
+
+template <typename T>
+int dmalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return kOk;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    set_error(std::string("hipMalloc(") + std::to_string(sizeof(T) * count) + " bytes): " +
+              hipGetErrorString(e));
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? kNoMemory : kHipError;
+  }
+  return kOk;
+}
+
+template <typename T>
+void dfree(T** p) {
+  if (*p) (void)hipFree((void*)*p);
+  *p = nullptr;
+}
+
+inline int pad64(int b) { return ((b + 63) / 64) * 64; }
+int cu_count();
+
+struct PrefixEntry {
+  long* d_prefix = nullptr;
+  int nitems = 0;
+  long total = 0;
+};
+
+}  // namespace edb
+
+struct edb_qcalc {
+  int N = 0, alpha = 0, filter = 1;
+  double L = 0.;
+  edb::Geometry g;
+  hipStream_t stream = nullptr;
+  edb::BetaTables tables;
+  bool tables_ready = false;
+  // plain FP64
+  double* d_plain = nullptr;
+  int ntile = 0;
+  int nslab = 16;
+  double* d_dup = nullptr;
+  double* d_part = nullptr;
+  // paired FP32
+  float* d_pair = nullptr;
+  int4* d_rowinfo = nullptr;
+  uint32_t* d_sched = nullptr;
+  std::vector<int4> rowinfo;
+  int nrows_half = 0;
+  long nblocks = 0;
+  std::map<std::pair<int, int>, edb::PrefixEntry> prefixes;
+  double* d_slots = nullptr;
+  size_t slots_count = 0;
+  int nwaves = 0;
+  std::mutex mu;
+};
+
+namespace edb {
+
+int ensure_tables(edb_qcalc* q);
+int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out);
+int ensure_slots(edb_qcalc* q, int nwaves);
+// Batched Q on device arrays already in the kernel layout (float2 [M^3][ld] in,
+// double2 [M^3][ld] out); events, if given, bracket the main kernel.
+int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, const float* d_fin,
+                   hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+int ensure_plain_scratch(edb_qcalc* q);
+// Plain FP64 Q of one column (centred FP64 input with stride fc_stride).
+int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, int q_stride,
+                 hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+
+}  // namespace edb

@@ -235,7 +235,8 @@ __global__ __launch_bounds__(512) void entropy_fix_kernel(EntropyFixArgs A) {
   const double H0 = A.H0[inst];
   double Hfix = H;
   int fixed = 0;
-  if (A.eligible && A.fix_mode != 0 && H - H0 > 1.0e-10) {
+  const bool elig = A.eligible && (!A.steps_done || A.steps_done[inst] > 0);
+  if (elig && A.fix_mode != 0 && H - H0 > 1.0e-10) {
     double avg = A.avg[inst];
     if (!A.avg_flag[inst]) {  // setAverage(Nf, fin), EntropyFix.h:225
       double s = 0.;
@@ -264,6 +265,47 @@ __global__ __launch_bounds__(512) void entropy_fix_kernel(EntropyFixArgs A) {
       A.hist_fixed[(long)rec * ld + inst] = fixed;
     }
   }
+}
+
+__global__ void move_col_kernel(long n, int ld, int src, int dst, double2* a) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) a[j * ld + dst] = a[j * ld + src];
+}
+
+__global__ void move_slot_scalars_kernel(int src, int dst, double* d0, double* d1, int* i0, int* i1) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (d0) d0[dst] = d0[src];
+  if (d1) d1[dst] = d1[src];
+  if (i0) i0[dst] = i0[src];
+  if (i1) i1[dst] = i1[src];
+}
+
+__global__ void incr_kernel(int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] += 1;
+}
+
+__global__ void extract_col_kernel(long n, int ld, int col, const double2* a, double* out) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = a[j * ld + col].x;
+}
+
+__global__ void reset_slots_kernel(int ld, const int* flag, double* H0, double* avg, int* avg_flag,
+                                   int* steps) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ld || !flag[i]) return;
+  H0[i] = 1.;
+  avg[i] = 1.;
+  avg_flag[i] = 0;
+  steps[i] = 0;
+}
+
+__global__ void stage_real_kernel(long n, int batch, int ld, const int* flag, const double* rows,
+                                  double2* F) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * ld) return;
+  long j = idx / ld, col = idx % ld;
+  if (col < batch && flag[col]) F[idx] = make_double2(rows[col * n + j], 0.);
 }
 
 __global__ void bump_kernel(int* p) {
@@ -443,6 +485,50 @@ int launch_select_cols(long n, int batch, int ld, const int* flag, const double*
 int launch_entropy_fix(const EntropyFixArgs& a, int batch, hipStream_t s) {
   EDB_REQUIRE(a.W != nullptr && a.N3 >= 1, "entropy_fix: missing work buffer");
   hipLaunchKernelGGL(entropy_fix_kernel, dim3(batch), dim3(kRedThreads), 0, s, a);
+  EDB_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+int launch_move_col(long n, int ld, int src, int dst, double* a, hipStream_t s) {
+  EDB_REQUIRE(src >= 0 && dst >= 0 && src < ld && dst < ld, "move_col: column out of range");
+  hipLaunchKernelGGL(move_col_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, n, ld, src, dst, (double2*)a);
+  EDB_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+int launch_move_slot_scalars(int src, int dst, double* d0, double* d1, int* i0, int* i1, hipStream_t s) {
+  hipLaunchKernelGGL(move_slot_scalars_kernel, dim3(1), dim3(64), 0, s, src, dst, d0, d1, i0, i1);
+  EDB_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+int launch_incr(int* p, int n, hipStream_t s) {
+  if (n <= 0) return kOk;
+  hipLaunchKernelGGL(incr_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+  EDB_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+int launch_extract_col(long n, int ld, int col, const double* a, double* out, hipStream_t s) {
+  EDB_REQUIRE(col >= 0 && col < ld, "extract_col: column out of range");
+  hipLaunchKernelGGL(extract_col_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, n, ld, col,
+                     (const double2*)a, out);
+  EDB_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+int launch_reset_slots(int ld, const int* flag, double* H0, double* avg, int* avg_flag, int* steps,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(reset_slots_kernel, dim3((ld + 255) / 256), dim3(256), 0, s, ld, flag, H0, avg,
+                     avg_flag, steps);
+  EDB_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+int launch_stage_real(long n, int batch, int ld, const int* flag, const double* rows, double* F,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(stage_real_kernel, dim3(nblk(n * ld, 256)), dim3(256), 0, s, n, batch, ld, flag, rows,
+                     (double2*)F);
   EDB_HIP_TRY(hipGetLastError());
   return kOk;
 }

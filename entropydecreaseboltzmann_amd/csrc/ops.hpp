@@ -32,6 +32,7 @@ struct EntropyFixArgs {
   const int* rec_ptr = nullptr;  // device counter of the history row
   int hist_cap = 0;
   int eligible = 0;
+  const int* steps_done = nullptr;  // per instance: eligible only when steps_done[i] > 0
   int fix_mode = 0;
   double eps = 1.0e-10;
   double vol = 0.;          // (2R)^3
@@ -55,6 +56,21 @@ int launch_select_cols(long n, int batch, int ld, const int* flag, const double*
                        hipStream_t s);
 int launch_entropy_fix(const EntropyFixArgs& a, int batch, hipStream_t s);
 int launch_bump(int* p, hipStream_t s);
+// Move column src -> dst of a [n][ld] complex array (slot compaction).
+int launch_move_col(long n, int ld, int src, int dst, double* a, hipStream_t s);
+// Per-slot scalars: dst <- src for the arrays given (nullable), and
+// steps[i] += 1 for i < batch (increment kernel).
+int launch_move_slot_scalars(int src, int dst, double* d0, double* d1, int* i0, int* i1, hipStream_t s);
+int launch_incr(int* p, int n, hipStream_t s);
+// For flagged slots: H0 = avg = 1, avg_flag = steps = 0 (a fresh EntropyFix
+// and driver state, EntropyFix.h:20-22, BKW.h:478).
+int launch_reset_slots(int ld, const int* flag, double* H0, double* avg, int* avg_flag, int* steps,
+                       hipStream_t s);
+// For flagged columns: F[j][col] = (rows[col*n + j], 0).
+int launch_stage_real(long n, int batch, int ld, const int* flag, const double* rows, double* F,
+                      hipStream_t s);
+// Real part of column col of a [n][ld] complex array into out[n].
+int launch_extract_col(long n, int ld, int col, const double* a, double* out, hipStream_t s);
 int launch_efix(int mode, long N3, double* d_f, double eps, double aim, double* d_st, hipStream_t s);
 int launch_moments(int N, int batch, int ld, double L, const double* F, double* out, hipStream_t s);
 int launch_bkw_error(int N, int batch, int ld, double L, double K, const double* F, double* out,

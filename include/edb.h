@@ -229,6 +229,39 @@ int edb_solver_bkw_error(edb_solver* s, double K, double* out);
 
 int edb_solver_destroy(edb_solver* s);
 
+/* ------------------------------------------- request server (serving) -- */
+/* Continuous batching of independent solves (RK4, dt, entropy fix NONE or
+ * APPLIED).  Requests of one (N, alpha) share a group: a cached collision
+ * kernel and `max_slots` device-resident instance slots on a stream of their
+ * own; finished slots are freed and waiting requests admitted at chunk
+ * boundaries.  No reference counterpart (the reference has no serving). */
+typedef struct {
+  double L;          /* velocity half-width */
+  int filter;        /* Jackson filter */
+  double dt;
+  int fix_mode;      /* EDB_FIX_NONE or EDB_FIX_APPLIED */
+  int max_slots;     /* slots per group (0: 256, 64 for N >= 27) */
+  int max_chunk;     /* RK4 steps per launch round (0: 16) */
+  long hbm_budget;   /* bytes for kernels + slots over all groups (0: unlimited) */
+} edb_server_params;
+
+typedef struct edb_server edb_server;
+
+int edb_server_create(const edb_server_params* p, edb_server** out);
+/* Queue one solve of f(t=0) = F0 (N^3 real on x_i = -L + i*2L/N) to t_end
+ * (the drivers' step count, do { t += dt } while (t < t_end)).  EDB_ENOMEM:
+ * its group can never fit the budget; EDB_EBUSY: not now (groups in use). */
+int edb_server_submit(edb_server* s, long id, int N, int alpha, double t_end, const double* F0);
+/* Advance the scheduler without blocking: retire finished chunks, admit
+ * waiting requests, launch the next chunk of each group.  *ready = number of
+ * results waiting to be fetched. */
+int edb_server_poll(edb_server* s, int* ready);
+/* Pop one finished solve: its id, N, steps, final t and real-space F. */
+int edb_server_fetch(edb_server* s, long* id, int* N, int* steps, double* t, double* F, long capacity);
+int edb_server_stats(const edb_server* s, long* submitted, long* finished, long* active, long* waiting,
+                     long* hbm_used);
+int edb_server_destroy(edb_server* s);
+
 #ifdef __cplusplus
 }
 #endif
