@@ -124,6 +124,98 @@ def profile_kernels(edb, N, batch):
     hip.hipFree(dst)
 
 
+def serve(edb, args):
+    """Serving run: requests arrive as a Poisson process at --rate for --duration
+    seconds, each drawing N from {16, 24, 32}, gamma from {0, 1}, t_end ~ U[1, 5]
+    and a mixed-Maxwellian initial f from its own seed; RK4 dt = 0.01 with the
+    applied entropy fix.  Latency = completion - arrival (wall clock)."""
+    L, dt = 8.0, 0.01
+    rng = np.random.default_rng(args.seed)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / args.rate)
+        if t > args.duration:
+            break
+        arrivals.append(t)
+    reqs = []
+    for i, ta in enumerate(arrivals):
+        reqs.append((i, ta, int(rng.choice([16, 24, 32])), int(rng.integers(0, 2)),
+                     float(rng.uniform(1.0, 5.0))))
+    srv = edb.Server(L=L, dt=dt, fix_mode=edb.FIX_APPLIED, max_chunk=8)
+    # warm the six groups (collision kernels built, buffers allocated) before the clock starts
+    warm = 0
+    for N in (16, 24, 32):
+        for g in (0, 1):
+            srv.submit(10 ** 9 + warm, N, g, dt, mixture(N, L, 7))
+            warm += 1
+    got = 0
+    while got < warm:
+        if srv.poll():
+            while True:
+                try:
+                    srv.fetch()
+                    got += 1
+                except edb.EdbError:
+                    break
+    steps_of = lambda te: int(np.ceil(te / dt - 1e-9))
+    done_t, lat, qevals = {}, [], 0
+    t0 = time.perf_counter()
+    nxt, backlog = 0, []
+    end_arrivals = t0 + args.duration
+    deadline = end_arrivals + args.drain_max
+    while True:
+        now = time.perf_counter()
+        while nxt < len(reqs) and t0 + reqs[nxt][1] <= now:
+            backlog.append(reqs[nxt])
+            nxt += 1
+        still = []
+        for (i, ta, N, g, te) in backlog:
+            st = srv.submit(i, N, g, te, mixture(N, L, args.seed * 100003 + i))
+            if st != 0:
+                still.append((i, ta, N, g, te))
+        backlog = still
+        if srv.poll():
+            while True:
+                try:
+                    rid, N, steps, tt, F = srv.fetch()
+                except edb.EdbError:
+                    break
+                tc = time.perf_counter()
+                lat.append(tc - (t0 + reqs[rid][1]))
+                qevals += 4 * steps
+                done_t[rid] = tc
+        if nxt >= len(reqs) and len(done_t) == len(reqs):
+            break
+        if now > deadline:
+            break
+    wall = time.perf_counter() - t0
+    lat = np.array(lat)
+    st = srv.stats()
+    # one-at-a-time FP64: plain Q-eval time per N times this mix's Q-evals, plus per-step overhead
+    plain = {}
+    for N in (16, 24, 32):
+        q = edb.QCalculator(N, 0, L)
+        q.closeFilter()
+        q.genB()
+        q.autotune(0)
+        plain[N] = q.time_plain(reps=3) * 1e-3
+        q.close()
+    seq = sum(4 * steps_of(te) * plain[N] for (_i, _ta, N, _g, te) in reqs)
+    out = {"metric": "serving: p50/p99 solve latency, throughput",
+           "rate_req_s": args.rate, "duration_s": args.duration, "requests": len(reqs),
+           "completed": int(len(lat)), "unfinished": int(len(reqs) - len(lat)),
+           "wall_s": round(wall, 2),
+           "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+           "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+           "throughput_req_s": round(len(lat) / wall, 2), "qevals_per_s": round(qevals / wall, 1),
+           "one_at_a_time_fp64_req_s": round(len(reqs) / seq, 3),
+           "one_at_a_time_note": "plain FP64 Q-evaluation time per N (autotuned) x 4 Q-evals per RK4 "
+                                 "step of each request, summed over this request mix",
+           "speedup_vs_one_at_a_time": round((len(lat) / wall) / (len(reqs) / seq), 1),
+           "server": st}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +226,13 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay each RK4 step from a HIP graph")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve", action="store_true",
+                    help="serving run: Poisson arrivals of mixed (N, gamma, t_end) solves")
+    ap.add_argument("--rate", type=float, default=200.0, help="requests per second (--serve)")
+    ap.add_argument("--duration", type=float, default=10.0, help="seconds of arrivals (--serve)")
+    ap.add_argument("--drain-max", type=float, default=60.0,
+                    help="seconds allowed after the arrivals stop (--serve)")
+    ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--profile-kernels", action="store_true",
                     help="no solver: one 256 MiB device copy (the reference transfer for PMC "
                          "scaling), then 3 batched and 3 plain Q launches at --N/--batch")
@@ -147,6 +246,9 @@ def main():
 
     if args.profile_kernels:
         profile_kernels(edb, args.N, args.batch)
+        return
+    if args.serve:
+        serve(edb, args)
         return
     N, L, dt, batch = args.N, 8.0, 0.01, args.batch
     M = edb.nmodes(N)

@@ -69,6 +69,7 @@ struct BatchedArgs {
   int Bp;
   double2* q;
   double2* slots;
+  int* wstart;  // [nwaves] item containing each wave's first step
   int hermitian;
 };
 
@@ -158,6 +159,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     else hi = mid;
   }
   int it = lo;
+  if (lane == 0) A.wstart[w] = it;
   long pos = a0;
   bool first = true;
   while (pos < b0) {
@@ -211,22 +213,27 @@ __device__ inline int wave_of(long total, int nwaves, long x) {
   return w;
 }
 
-__global__ void calq_fixup_kernel(BatchedArgs A, int N, int KS) {
+// One block per interior wave boundary w (1..nwaves-1).  The item that
+// contains the boundary strictly inside it (recorded by the main kernel as
+// wave w's first item) was split between waves; the block of the first such
+// boundary in the item sums that item's partial slots in wave order, its four
+// waves splitting the k3 entries.  Items with no interior boundary were
+// written by their wave.
+__global__ __launch_bounds__(256) void calq_fixup_kernel(BatchedArgs A, int N, int KS) {
   const int n = (N - 1) / 2, M = 2 * n + 1, KPT = (M + KS - 1) / KS;
-  const int it = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (it >= A.nitems) return;
+  const int wb = blockIdx.x + 1;
+  const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+  if (wb >= A.nwaves) return;
+  const long p = range_start(A.total, A.nwaves, wb);
+  const int it = A.wstart[wb];
   const long lo = A.prefix[it], hi = A.prefix[it + 1];
-  if (hi <= lo) return;
+  if (lo == p || range_start(A.total, A.nwaves, wb - 1) > lo) return;
   const int wf = wave_of(A.total, A.nwaves, lo), wl = wave_of(A.total, A.nwaves, hi - 1);
-  if (wf == wl && range_start(A.total, A.nwaves, wf) <= lo &&
-      range_start(A.total, A.nwaves, wf + 1) >= hi)
-    return;  // one wave covered the whole item and wrote it
   const int r = it / (KS * A.groups), h = (it / A.groups) % KS, g = it % A.groups;
   const int4 ri = A.rowinfo[r];
   const bool mirror = A.hermitian && !(ri.x == n && ri.y == n);
   const long col = (long)g * 64 + lane;
-  for (int j = 0; j < KPT; ++j) {
+  for (int j = sub; j < KPT; j += 4) {
     const int k3 = h * KPT + j;
     if (k3 >= M) break;
     double sr = 0., si = 0.;
@@ -270,6 +277,13 @@ BatchedFn lookup(int N, int ks) {
 }  // namespace
 
 int batched_ksplit(int N) { return ksplit_of(N); }
+
+// Doubles of the stream-K partial slots of nwaves waves (2 slots of KPT x 64
+// complex each); the per-wave start items follow them as ints.
+size_t batched_slot_doubles(int N, int nwaves) {
+  const int M = 2 * ((N - 1) / 2) + 1, ks = ksplit_of(N), kpt = (M + ks - 1) / ks;
+  return (size_t)nwaves * 2 * kpt * 64 * 2;
+}
 bool batched_supported(int N) { return lookup(N, ksplit_of(N)) != nullptr; }
 
 // Host-side schedule: rows ordered Hermitian half first; per row the l-rows
@@ -332,14 +346,17 @@ int launch_calq_batched_impl(int N, const float* d_B, const uint32_t* d_sched, c
   A.B = d_B; A.sched = d_sched; A.rowinfo = d_rowinfo; A.prefix = d_prefix;
   A.nitems = nitems; A.groups = groups; A.total = total; A.nwaves = nwaves;
   A.fin = (const float2*)d_fin; A.Bp = Bp; A.q = (double2*)d_q; A.slots = (double2*)d_slots;
+  A.wstart = (int*)(d_slots + batched_slot_doubles(N, nwaves));
   A.hermitian = hermitian;
   const int wpb = 4;
   if (ev0) EDB_HIP_TRY(hipEventRecord(ev0, s));
   hipLaunchKernelGGL(fn, dim3((nwaves + wpb - 1) / wpb), dim3(64 * wpb), 0, s, A);
   EDB_HIP_TRY(hipGetLastError());
   if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
-  hipLaunchKernelGGL(calq_fixup_kernel, dim3(nitems), dim3(64), 0, s, A, N, ks);
-  EDB_HIP_TRY(hipGetLastError());
+  if (nwaves > 1) {
+    hipLaunchKernelGGL(calq_fixup_kernel, dim3(nwaves - 1), dim3(256), 0, s, A, N, ks);
+    EDB_HIP_TRY(hipGetLastError());
+  }
   return kOk;
 }
 

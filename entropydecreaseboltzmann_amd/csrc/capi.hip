@@ -63,9 +63,7 @@ int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out) {
 }
 
 int ensure_slots(edb_qcalc* q, int nwaves) {
-  const int ks = batched_ksplit(q->N);
-  const int kpt = (q->g.M + ks - 1) / ks;
-  size_t need = (size_t)nwaves * 2 * kpt * 64 * 2;
+  size_t need = batched_slot_doubles(q->N, nwaves) + (size_t)(nwaves + 1) / 2;
   if (need <= q->slots_count) return kOk;
   dfree(&q->d_slots);
   q->slots_count = 0;
@@ -648,6 +646,7 @@ int post_step(edb_solver* s, int eligible) {
   a.ld = ld;
   a.F = (double2*)s->Fr;
   a.W = s->S1;
+  a.part = s->S1 + N3 * ld;  // second half of S1 (W uses the first)
   a.H0 = s->H0;
   a.avg = s->avg;
   a.avg_flag = s->avg_flag;
@@ -665,7 +664,8 @@ int post_step(edb_solver* s, int eligible) {
   a.vol_over_N3 = std::pow(2 * R, 3.) / (double)N3;
   EDB_TRY(launch_entropy_fix(a, batch, s->stream));
   if (s->p.fix_mode == EDB_FIX_APPLIED && eligible) {
-    EDB_TRY(launch_fft3d(N, N, N, batch, ld, -1, 1.0, s->Fr, s->Ft, s->S2, s->stream));
+    // only the fixed instances are transformed back (the others keep Fh)
+    EDB_TRY(launch_fft3d(N, N, N, batch, ld, -1, 1.0, s->Fr, s->Ft, s->S2, s->stream, s->fixed));
     EDB_TRY(launch_select_cols(N3, batch, ld, s->fixed, s->Ft, s->Fh, s->stream));
   }
   EDB_TRY(launch_bump(s->d_rec, s->stream));
@@ -806,6 +806,7 @@ int init_common(edb_solver* s, double T0) {
     a.ld = ld;
     a.F = (double2*)s->Fr;
     a.W = s->S1;
+    a.part = s->S1 + s->g.N3 * ld;
     a.H0 = s->H0;
     a.avg = s->avg;
     a.avg_flag = s->avg_flag;

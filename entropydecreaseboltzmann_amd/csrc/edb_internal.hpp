@@ -108,7 +108,8 @@ int launch_reduce_plain(int N, const double* d_part, int nslab, double* d_q, int
 
 // FFT (fft3d.hip): lanes over instances, DFT passes along each axis.
 int launch_fft3d(int Nx, int Ny, int Nz, int batch, int Bp, int sign, double scale,
-                 const double* d_in, double* d_out, double* d_tmp, hipStream_t s);
+                 const double* d_in, double* d_out, double* d_tmp, hipStream_t s,
+                 const int* mask = nullptr);
 int fft_prepare(int N);
 
 }  // namespace edb

@@ -22,11 +22,19 @@ namespace {
 
 typedef const double __attribute__((address_space(4)))* cdouble_p;
 
+// Symmetric form of the length-NA DFT: with w_r = c_r + i s_r the twiddle
+// table (s carries the transform's sign), inputs j and NA-j share c and have
+// opposite s, so with S_j = x_j + x_{NA-j} and D_j = x_j - x_{NA-j},
+//   A_k = x_0 + (-1)^k x_{NA/2} + sum_{j=1..H} c_{jk} S_j,   B_k = sum_j s_{jk} D_j,
+//   X_k = A_k + i B_k,   X_{NA-k} = A_k - i B_k          (H = (NA-1)/2),
+// a quarter of the multiply-adds of the direct sum.
 template <int NA>
-__global__ __launch_bounds__(256) void dft_pass_kernel(const double2* __restrict__ in, double2* out,
-                                                       int ld, int groups, long npencil,
-                                                       long stride, int inner, long outer_stride,
-                                                       const double* __restrict__ tw_, double scale) {
+__global__ __launch_bounds__(256) void dft_pass_kernel(const double2* in, double2* out, int ld,
+                                                       int groups, long npencil, long stride, int inner,
+                                                       long outer_stride, const double* __restrict__ tw_,
+                                                       double scale, const int* __restrict__ mask) {
+  constexpr int H = (NA - 1) / 2;
+  constexpr bool EVEN = (NA % 2) == 0;
   const cdouble_p tw = (cdouble_p)tw_;  // [2*NA]: cos, sign*sin of 2 pi r/NA
   const int lane = threadIdx.x & 63;
   const long wid = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -36,28 +44,63 @@ __global__ __launch_bounds__(256) void dft_pass_kernel(const double2* __restrict
   // pencil p: index base = (p / inner) * outer_stride + (p % inner)
   const long base = (p / inner) * outer_stride + (p % inner);
   const long col = (long)g * 64 + lane;
-  double2 x[NA];
+  if (mask && __ballot(mask[col] != 0) == 0) return;  // no instance of this wave is wanted
+  const double2* src = in + base * ld + col;
+  const double2 x0 = src[0];
+  double2 xh = make_double2(0., 0.);
+  if (EVEN) xh = src[(long)(NA / 2) * stride * ld];
+  double2 S[H > 0 ? H : 1], D[H > 0 ? H : 1];
 #pragma unroll
-  for (int j = 0; j < NA; ++j) x[j] = in[(base + j * stride) * ld + col];
+  for (int j = 1; j <= H; ++j) {
+    const double2 a = src[(long)j * stride * ld], b = src[(long)(NA - j) * stride * ld];
+    S[j - 1] = make_double2(a.x + b.x, a.y + b.y);
+    D[j - 1] = make_double2(a.x - b.x, a.y - b.y);
+  }
+  double2* dst = out + base * ld + col;
+  {
+    double sr = x0.x + xh.x, si = x0.y + xh.y;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      sr += S[j].x;
+      si += S[j].y;
+    }
+    dst[0] = make_double2(sr * scale, si * scale);
+  }
 #pragma unroll 1
-  for (int k = 0; k < NA; ++k) {
-    double sr = 0., si = 0.;
+  for (int k = 1; k <= H; ++k) {
+    const double sg = (k & 1) ? -1. : 1.;
+    double ar = x0.x + sg * xh.x, ai = x0.y + sg * xh.y, br = 0., bi = 0.;
     int r = 0;  // (j*k) mod NA, wave-uniform
 #pragma unroll
-    for (int j = 0; j < NA; ++j) {
-      const double c = tw[2 * r], s = tw[2 * r + 1];
-      sr += x[j].x * c - x[j].y * s;
-      si += x[j].x * s + x[j].y * c;
+    for (int j = 0; j < H; ++j) {
       r += k;
       if (r >= NA) r -= NA;
+      const double c = tw[2 * r], sn = tw[2 * r + 1];
+      ar += S[j].x * c;
+      ai += S[j].y * c;
+      br += D[j].x * sn;
+      bi += D[j].y * sn;
     }
-    out[(base + k * stride) * ld + col] = make_double2(sr * scale, si * scale);
+    dst[(long)k * stride * ld] = make_double2((ar - bi) * scale, (ai + br) * scale);
+    dst[(long)(NA - k) * stride * ld] = make_double2((ar + bi) * scale, (ai - br) * scale);
+  }
+  if (EVEN) {  // k = NA/2: c_{j,NA/2} = (-1)^j, s = 0
+    const double sg = ((NA / 2) & 1) ? -1. : 1.;
+    double sr = x0.x + sg * xh.x, si = x0.y + sg * xh.y;
+#pragma unroll
+    for (int j = 0; j < H; ++j) {
+      const double c = ((j + 1) & 1) ? -1. : 1.;
+      sr += S[j].x * c;
+      si += S[j].y * c;
+    }
+    dst[(long)(NA / 2) * stride * ld] = make_double2(sr * scale, si * scale);
   }
 }
 
 __global__ void dft_pass_generic(const double2* __restrict__ in, double2* __restrict__ out, int ld,
                                  int groups, long npencil, long stride, int inner, long outer_stride,
-                                 int NA, const double* __restrict__ tw, double scale) {
+                                 int NA, const double* __restrict__ tw, double scale,
+                                 const int* __restrict__ mask) {
   const int lane = threadIdx.x & 63;
   const long wid = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const long p = wid / groups;
@@ -65,6 +108,7 @@ __global__ void dft_pass_generic(const double2* __restrict__ in, double2* __rest
   if (p >= npencil) return;
   const long base = (p / inner) * outer_stride + (p % inner);
   const long col = (long)g * 64 + lane;
+  if (mask && __ballot(mask[col] != 0) == 0) return;
   for (int k = 0; k < NA; ++k) {
     double sr = 0., si = 0.;
     for (int j = 0; j < NA; ++j) {
@@ -79,7 +123,7 @@ __global__ void dft_pass_generic(const double2* __restrict__ in, double2* __rest
 }
 
 typedef void (*PassFn)(const double2*, double2*, int, int, long, long, int, long, const double*,
-                       double);
+                       double, const int*);
 
 PassFn lookup_pass(int NA) {
   switch (NA) {
@@ -113,8 +157,11 @@ int twiddles(int NA, int sign, const double** out) {
   std::vector<double> h(2 * NA);
   for (int r = 0; r < NA; ++r) {
     double a = 2. * M_PI * (double)r / (double)NA;
-    h[2 * r] = cos(a);
-    h[2 * r + 1] = sign * sin(a);
+    double c = cos(a), sn = sin(a);
+    if (4 * r == NA || 4 * r == 3 * NA) c = 0.;  // exact zeros at the quarter turns
+    if (2 * r == NA) sn = 0.;
+    h[2 * r] = c;
+    h[2 * r + 1] = sign * sn;
   }
   double* d = nullptr;
   EDB_HIP_TRY(hipMalloc(&d, sizeof(double) * 2 * NA));
@@ -125,7 +172,7 @@ int twiddles(int NA, int sign, const double** out) {
 }
 
 int run_pass(int NA, const double2* in, double2* out, int ld, int groups, long npencil, long stride,
-             int inner, long outer_stride, int sign, double scale, hipStream_t s) {
+             int inner, long outer_stride, int sign, double scale, const int* mask, hipStream_t s) {
   const double* tw = nullptr;
   EDB_TRY(twiddles(NA, sign, &tw));
   long waves = npencil * groups;
@@ -134,11 +181,11 @@ int run_pass(int NA, const double2* in, double2* out, int ld, int groups, long n
   PassFn fn = lookup_pass(NA);
   if (fn) {
     hipLaunchKernelGGL(fn, grid, block, 0, s, in, out, ld, groups, npencil, stride, inner,
-                       outer_stride, tw, scale);
+                       outer_stride, tw, scale, mask);
   } else {
     EDB_REQUIRE(in != out, "fft3d: in-place pass needs a compiled length");
     hipLaunchKernelGGL(dft_pass_generic, grid, block, 0, s, in, out, ld, groups, npencil, stride,
-                       inner, outer_stride, NA, tw, scale);
+                       inner, outer_stride, NA, tw, scale, mask);
   }
   EDB_HIP_TRY(hipGetLastError());
   return kOk;
@@ -156,9 +203,11 @@ int fft_prepare(int N) {
 
 // sign -1: forward; +1: backward.  scale multiplies the final pass.  d_tmp
 // (same size as d_out) is needed only when an axis length has no compiled
-// pass (in-place passes need the register version).
+// pass (in-place passes need the register version).  With a mask ([Bp] ints),
+// waves whose 64 instances all have mask 0 are skipped and those columns of
+// d_out are left undefined.
 int launch_fft3d(int Nx, int Ny, int Nz, int batch, int Bp, int sign, double scale,
-                 const double* d_in, double* d_out, double* d_tmp, hipStream_t s) {
+                 const double* d_in, double* d_out, double* d_tmp, hipStream_t s, const int* mask) {
   EDB_REQUIRE(Nx >= 1 && Ny >= 1 && Nz >= 1 && Nx <= kMaxN && Ny <= kMaxN && Nz <= kMaxN,
               "fft3d: each length must be in [1, 64]");
   EDB_REQUIRE(batch >= 1 && Bp % 64 == 0 && batch <= Bp, "fft3d: bad batch / leading dimension");
@@ -183,7 +232,7 @@ int launch_fft3d(int Nx, int Ny, int Nz, int batch, int Bp, int sign, double sca
       dst = tmp;
     }
     EDB_TRY(run_pass(ax[a].NA, src, dst, Bp, groups, ax[a].npencil, ax[a].stride, ax[a].inner,
-                     ax[a].outer, sign, sc, s));
+                     ax[a].outer, sign, sc, mask, s));
     if (dst == tmp) {
       EDB_HIP_TRY(hipMemcpyAsync(out, tmp, sizeof(double2) * (long)Nx * Ny * Nz * Bp,
                                  hipMemcpyDeviceToDevice, s));

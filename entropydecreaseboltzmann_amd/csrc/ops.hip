@@ -6,6 +6,7 @@
 // per instance, so every elementwise pass is coalesced across instances.
 // Per-instance reductions use one workgroup per instance and a fixed-shape
 // tree, so results are deterministic.
+#include <algorithm>
 #include <cmath>
 
 #include "edb_internal.hpp"
@@ -211,14 +212,32 @@ __device__ void fixsol_block(double* f, long N3, double aim, double eps, double 
   __syncthreads();
 }
 
+// Coalesced first pass: lane = instance, wave = (instance group, grid chunk);
+// each lane sums flogf over its chunk of grid points.
+__global__ __launch_bounds__(256) void entropy_part_kernel(long N3, int ld, int groups, int chunks,
+                                                           const double2* __restrict__ F,
+                                                           double* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const long wid = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (wid >= (long)groups * chunks) return;
+  const int c = (int)(wid / groups), g = (int)(wid % groups);
+  const long col = (long)g * 64 + lane;
+  const long per = (N3 + chunks - 1) / chunks;
+  const long g0 = (long)c * per, g1 = g0 + per < N3 ? g0 + per : N3;
+  double acc = 0.;
+  for (long q = g0; q < g1; ++q) acc += flogf_dev(F[q * ld + col].x);
+  part[(long)c * ld + col] = acc;
+}
+
 // Per instance (one 512-thread block each): calEntropy of Re F
 // (BKW.h:140-152); if eligible and H - H0 > 1e-10, FixSol on Re F with target
 // H0/(2R)^3 (the drivers' call, BKW.h:229-233, 502-506; the average cached per
 // instance at its first fix, EntropyFix.h:225), then calEntropy of the fixed
-// F.  Re F is staged in the instance's contiguous work row W.  Mode 2 writes
+// F.  H is summed from the first pass's partials; Re F is staged in the
+// instance's contiguous work row W only when the fix runs.  Mode 2 writes
 // the fixed F back (imaginary part 0) and flags the instance for the forward
 // transform.  Records H, Hfix and whether a fix ran.
-__global__ __launch_bounds__(512) void entropy_fix_kernel(EntropyFixArgs A) {
+__global__ __launch_bounds__(512) void entropy_fix_kernel(EntropyFixArgs A, int chunks) {
   __shared__ double sh[kRedThreads];
   const int inst = blockIdx.x;
   const int t = threadIdx.x;
@@ -226,17 +245,15 @@ __global__ __launch_bounds__(512) void entropy_fix_kernel(EntropyFixArgs A) {
   const int ld = A.ld;
   double* w = A.W + (long)inst * N3;
   double part = 0.;
-  for (long g = t; g < N3; g += kRedThreads) {
-    const double f = A.F[g * ld + inst].x;
-    w[g] = f;
-    part += flogf_dev(f);
-  }
+  for (int c = t; c < chunks; c += kRedThreads) part += A.part[(long)c * ld + inst];
   const double H = block_sum(part, sh) * A.vol_over_N3;
   const double H0 = A.H0[inst];
   double Hfix = H;
   int fixed = 0;
   const bool elig = A.eligible && (!A.steps_done || A.steps_done[inst] > 0);
   if (elig && A.fix_mode != 0 && H - H0 > 1.0e-10) {
+    for (long g = t; g < N3; g += kRedThreads) w[g] = A.F[g * ld + inst].x;
+    __syncthreads();
     double avg = A.avg[inst];
     if (!A.avg_flag[inst]) {  // setAverage(Nf, fin), EntropyFix.h:225
       double s = 0.;
@@ -482,9 +499,21 @@ int launch_select_cols(long n, int batch, int ld, const int* flag, const double*
   return kOk;
 }
 
+int entropy_chunks(long N3, int ld) {
+  const int groups = (ld + 63) / 64;
+  long c = (4096 + groups - 1) / groups;  // about 4096 waves in the first pass
+  return (int)std::max(1L, std::min(c, N3));
+}
+
 int launch_entropy_fix(const EntropyFixArgs& a, int batch, hipStream_t s) {
-  EDB_REQUIRE(a.W != nullptr && a.N3 >= 1, "entropy_fix: missing work buffer");
-  hipLaunchKernelGGL(entropy_fix_kernel, dim3(batch), dim3(kRedThreads), 0, s, a);
+  EDB_REQUIRE(a.W != nullptr && a.part != nullptr && a.N3 >= 1, "entropy_fix: missing work buffer");
+  EDB_REQUIRE(a.ld % 64 == 0 && batch <= a.ld, "entropy_fix: bad leading dimension");
+  const int groups = (batch + 63) / 64, chunks = entropy_chunks(a.N3, a.ld);
+  const long waves = (long)groups * chunks;
+  hipLaunchKernelGGL(entropy_part_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a.N3, a.ld,
+                     groups, chunks, (const double2*)a.F, a.part);
+  EDB_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(entropy_fix_kernel, dim3(batch), dim3(kRedThreads), 0, s, a, chunks);
   EDB_HIP_TRY(hipGetLastError());
   return kOk;
 }

@@ -21,6 +21,7 @@ struct EntropyFixArgs {
   int ld = 0;
   double2* F = nullptr;      // [N^3][ld] complex grid, real part is f
   double* W = nullptr;       // [ld][N^3] work rows (Re F staged per instance)
+  double* part = nullptr;    // [entropy_chunks(N3, ld)][ld] partial entropy sums
   double* H0 = nullptr;      // [ld] previous-step entropy (updated to Hfix)
   double* avg = nullptr;     // [ld] EntropyFix average
   int* avg_flag = nullptr;   // [ld]
@@ -38,6 +39,10 @@ struct EntropyFixArgs {
   double vol = 0.;          // (2R)^3
   double vol_over_N3 = 0.;  // (2R)^3 / N^3
 };
+
+// Number of grid chunks of the coalesced entropy pass (the partial-sum
+// buffer holds chunks * ld doubles, never more than N3 * ld).
+int entropy_chunks(long N3, int ld);
 
 int launch_combine(int op, int N, int batch, int ld, double* out, const double* X, const double* Y,
                    const double* Q0, const double* Q1, const double* Q2, const double* Q3, double c0,
@@ -78,6 +83,7 @@ int launch_bkw_error(int N, int batch, int ld, double L, double K, const double*
 
 // calq_batched.hip
 int batched_ksplit(int N);
+size_t batched_slot_doubles(int N, int nwaves);
 bool batched_supported(int N);
 int build_pair_plan(int N, std::vector<int4>* rowinfo, std::vector<uint32_t>* sched, int* nrows_half);
 int batched_prefix(const std::vector<int4>& rowinfo, int nrows, int groups, int ks,

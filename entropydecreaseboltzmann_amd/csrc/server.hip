@@ -61,7 +61,7 @@ struct Group {
   hipEvent_t ev = nullptr;
   bool busy = false;
   int chunk = 0;
-  double *Fh = nullptr, *Fr = nullptr, *Ft = nullptr, *W = nullptr, *Q[4] = {};
+  double *Fh = nullptr, *Fr = nullptr, *Ft = nullptr, *W = nullptr, *part = nullptr, *Q[4] = {};
   float* in = nullptr;
   double *H0 = nullptr, *avg = nullptr, *out = nullptr, *adm = nullptr;
   int *avg_flag = nullptr, *fixed = nullptr, *steps_done = nullptr, *newflag = nullptr;
@@ -80,7 +80,7 @@ long group_bytes(int N, int cap) {
   build_pair_plan(N, &ri, &sc, &nh);
   long beta = (long)sc.size() * g.M * g.M * 4;
   long per_slot = g.N3 * 16 * 3 + g.N3 * 8 + g.NQ * 16 * 4 + g.NQ * 8 + g.N3 * 8 * 2 + 64;
-  return beta + per_slot * cap;
+  return beta + per_slot * cap + (long)entropy_chunks(g.N3, cap) * cap * 8;
 }
 
 }  // namespace
@@ -101,6 +101,7 @@ void free_group(Group* g) {
   dfree(&g->Fr);
   dfree(&g->Ft);
   dfree(&g->W);
+  dfree(&g->part);
   for (auto& q : g->Q) dfree(&q);
   dfree(&g->in);
   dfree(&g->H0);
@@ -133,6 +134,7 @@ int make_group(edb_server* s, int N, int alpha, Group** out) {
   if (st == kOk) st = dmalloc(&g->Fr, grid);
   if (st == kOk) st = dmalloc(&g->Ft, grid);
   if (st == kOk) st = dmalloc(&g->W, (size_t)g->g.N3 * g->cap);
+  if (st == kOk) st = dmalloc(&g->part, (size_t)entropy_chunks(g->g.N3, g->cap) * g->cap);
   for (int i = 0; i < 4 && st == kOk; ++i) st = dmalloc(&g->Q[i], modes);
   if (st == kOk) st = dmalloc(&g->in, modes);
   if (st == kOk) st = dmalloc(&g->H0, (size_t)g->cap);
@@ -220,6 +222,7 @@ int rk4_step(Group* g, const edb_server_params& p) {
   a.ld = ld;
   a.F = (double2*)g->Fr;
   a.W = g->W;
+  a.part = g->part;
   a.H0 = g->H0;
   a.avg = g->avg;
   a.avg_flag = g->avg_flag;
@@ -231,7 +234,7 @@ int rk4_step(Group* g, const edb_server_params& p) {
   a.vol_over_N3 = a.vol / (double)N3;
   EDB_TRY(launch_entropy_fix(a, b, g->stream));
   if (p.fix_mode == EDB_FIX_APPLIED) {
-    EDB_TRY(launch_fft3d(N, N, N, b, ld, -1, 1.0, g->Fr, g->Ft, nullptr, g->stream));
+    EDB_TRY(launch_fft3d(N, N, N, b, ld, -1, 1.0, g->Fr, g->Ft, nullptr, g->stream, g->fixed));
     EDB_TRY(launch_select_cols(N3, b, ld, g->fixed, g->Ft, g->Fh, g->stream));
   }
   EDB_TRY(launch_incr(g->steps_done, b, g->stream));
