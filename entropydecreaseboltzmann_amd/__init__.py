@@ -122,6 +122,11 @@ class QCalculator:
               "autotune")
         return {"batched_waves": bw.value, "plain_slabs": ps.value}
 
+    def set_conservative(self, on=True):
+        """Moment correction of every Q this calculator returns (zero discrete
+        mass, momentum and energy moments); no reference counterpart."""
+        check(load().edb_qcalc_set_conservative(self._h, int(bool(on))), "set_conservative")
+
     @property
     def handle(self):
         return self._h
@@ -194,13 +199,14 @@ class Solver:
 
     def __init__(self, N, alpha=0, L=8.0, filter=False, batch=1, precision=PREC_FP64_PLAIN,
                  scheme="RK4", dt=0.01, beta=0.0, fix_mode=FIX_NONE, hermitian=True, history=0,
-                 qcalc=None):
+                 qcalc=None, conserve=False):
         p = _lib.SolverParams()
         p.N, p.alpha, p.L, p.filter = int(N), int(alpha), float(L), int(bool(filter))
         p.batch, p.precision = int(batch), int(precision)
         p.scheme = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
         p.dt, p.beta, p.fix_mode = float(dt), float(beta), int(fix_mode)
         p.hermitian, p.history = int(bool(hermitian)), int(history)
+        p.conserve = int(bool(conserve))
         self.p = p
         self.N, self.batch = p.N, p.batch
         self._h = ctypes.c_void_p()
@@ -289,10 +295,11 @@ class Server:
     """Continuous batching of independent RK4 solves (include/edb.h, edb_server_*)."""
 
     def __init__(self, L=8.0, filter=False, dt=0.01, fix_mode=FIX_APPLIED, max_slots=0, max_chunk=0,
-                 hbm_budget=0):
+                 hbm_budget=0, conserve=False):
         p = _lib.ServerParams()
         p.L, p.filter, p.dt, p.fix_mode = float(L), int(bool(filter)), float(dt), int(fix_mode)
         p.max_slots, p.max_chunk, p.hbm_budget = int(max_slots), int(max_chunk), int(hbm_budget)
+        p.conserve = int(bool(conserve))
         self.p = p
         self._h = ctypes.c_void_p()
         check(load().edb_server_create(ctypes.byref(p), ctypes.byref(self._h)), "edb_server_create")

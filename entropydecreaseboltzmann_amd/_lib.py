@@ -43,13 +43,13 @@ class SolverParams(ctypes.Structure):
                 ("precision", ctypes.c_int), ("scheme", ctypes.c_int),
                 ("dt", ctypes.c_double), ("beta", ctypes.c_double),
                 ("fix_mode", ctypes.c_int), ("hermitian", ctypes.c_int),
-                ("history", ctypes.c_int)]
+                ("history", ctypes.c_int), ("conserve", ctypes.c_int)]
 
 
 class ServerParams(ctypes.Structure):
     _fields_ = [("L", ctypes.c_double), ("filter", ctypes.c_int), ("dt", ctypes.c_double),
                 ("fix_mode", ctypes.c_int), ("max_slots", ctypes.c_int), ("max_chunk", ctypes.c_int),
-                ("hbm_budget", ctypes.c_long)]
+                ("hbm_budget", ctypes.c_long), ("conserve", ctypes.c_int)]
 
 
 _SIGS = {
@@ -72,6 +72,7 @@ _SIGS = {
     "edb_qcalc_time_plain": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "edb_qcalc_tune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "edb_qcalc_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
+    "edb_qcalc_set_conservative": (ctypes.c_int, [_vp, ctypes.c_int]),
     "edb_qcalc_destroy": (ctypes.c_int, [_vp]),
     "edb_fft3d_fft": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "edb_fft3d_ifft": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),

@@ -72,6 +72,17 @@ int ensure_slots(edb_qcalc* q, int nwaves) {
   return kOk;
 }
 
+int ensure_moment_table(edb_qcalc* q) {
+  if (q->d_mom) return kOk;
+  std::vector<double> tab;
+  int E = 0;
+  EDB_TRY(build_moment_table(q->N, q->L, &tab, &E));
+  EDB_TRY(dmalloc(&q->d_mom, tab.size()));
+  EDB_HIP_TRY(hipMemcpy(q->d_mom, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+  q->mom_E = E;
+  return kOk;
+}
+
 // Batched Q on device arrays already in the kernel layout.
 int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, const float* d_fin,
                    hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
@@ -216,6 +227,7 @@ int edb_qcalc_calQ_device(edb_qcalc* q, double* d_qout, const double* d_fin, voi
   int st = dmalloc(&qc, (size_t)q->g.NQ * 2);
   if (st == kOk) st = launch_pack(q->N, 1, 1, false, fc, d_fin, nullptr, 0., s);
   if (st == kOk) st = plain_column(q, fc, 1, qc, 1, s);
+  if (st == kOk && q->conserve) st = launch_moment_correct(q->d_mom, q->mom_E, 1, 1, qc, s);
   if (st == kOk) st = launch_unpack(q->N, 1, 1, d_qout, qc, s);
   if (st == kOk && hipStreamSynchronize(s) != hipSuccess) {
     set_error("calQ_device: stream synchronisation failed");
@@ -245,7 +257,9 @@ int edb_qcalc_calQ_batched_device(edb_qcalc* q, int batch, int ld, int hermitian
                                   const float* d_fin, void* stream) {
   EDB_REQUIRE(q && d_qout && d_fin, "calQ_batched_device: null argument");
   hipStream_t s = stream ? (hipStream_t)stream : q->stream;
-  return batched_device(q, batch, ld, hermitian, d_qout, d_fin, s);
+  EDB_TRY(batched_device(q, batch, ld, hermitian, d_qout, d_fin, s));
+  if (q->conserve) EDB_TRY(launch_moment_correct(q->d_mom, q->mom_E, batch, ld, d_qout, s));
+  return EDB_OK;
 }
 
 int edb_qcalc_calQ_batched(edb_qcalc* q, int batch, int hermitian, double* qout, const double* fin) {
@@ -267,6 +281,7 @@ int edb_qcalc_calQ_batched(edb_qcalc* q, int batch, int hermitian, double* qout,
   if (st == kOk) st = launch_gather_cols(N3, batch, ld, dhost, dgrid, s);
   if (st == kOk) st = launch_pack(q->N, batch, ld, true, din, dgrid, nullptr, 0., s);
   if (st == kOk) st = batched_device(q, batch, ld, hermitian, dq, din, s);
+  if (st == kOk && q->conserve) st = launch_moment_correct(q->d_mom, q->mom_E, batch, ld, dq, s);
   if (st == kOk) st = launch_unpack(q->N, batch, ld, dgrid, dq, s);
   if (st == kOk) st = launch_scatter_cols(N3, batch, ld, dgrid, dhost, s);
   if (st == kOk && hipStreamSynchronize(s) != hipSuccess) {
@@ -441,9 +456,18 @@ int edb_qcalc_autotune(edb_qcalc* q, int batch, int* batched_waves, int* plain_s
   return EDB_OK;
 }
 
+int edb_qcalc_set_conservative(edb_qcalc* q, int on) {
+  EDB_REQUIRE(q, "null handle");
+  std::lock_guard<std::mutex> lk(q->mu);
+  if (on) EDB_TRY(ensure_moment_table(q));
+  q->conserve = on ? 1 : 0;
+  return EDB_OK;
+}
+
 int edb_qcalc_destroy(edb_qcalc* q) {
   if (!q) return EDB_OK;
   if (q->stream) (void)hipStreamSynchronize(q->stream);
+  dfree(&q->d_mom);
   dfree(&q->d_plain);
   dfree(&q->d_dup);
   dfree(&q->d_part);
@@ -618,13 +642,15 @@ int qeval(edb_solver* s, const double* X, const double* Qprev, double c0, double
   if (batched(s)) {
     hipEvent_t e0, e1;
     EDB_TRY(next_events(s, &e0, &e1));
-    return batched_device(s->qc, batch, ld, s->p.hermitian, Qout, (const float*)s->in, s->stream, e0, e1);
+    EDB_TRY(batched_device(s->qc, batch, ld, s->p.hermitian, Qout, (const float*)s->in, s->stream, e0, e1));
+  } else {
+    for (int i = 0; i < batch; ++i) {
+      hipEvent_t e0, e1;
+      EDB_TRY(next_events(s, &e0, &e1));
+      EDB_TRY(plain_column(s->qc, (const double*)s->in + 2 * i, ld, Qout + 2 * i, ld, s->stream, e0, e1));
+    }
   }
-  for (int i = 0; i < batch; ++i) {
-    hipEvent_t e0, e1;
-    EDB_TRY(next_events(s, &e0, &e1));
-    EDB_TRY(plain_column(s->qc, (const double*)s->in + 2 * i, ld, Qout + 2 * i, ld, s->stream, e0, e1));
-  }
+  if (s->p.conserve) EDB_TRY(launch_moment_correct(s->qc->d_mom, s->qc->mom_E, batch, ld, Qout, s->stream));
   return kOk;
 }
 
@@ -774,6 +800,7 @@ int run_one(edb_solver* s) {
 // captured into a HIP graph (no allocation during capture).
 int prepare_buffers(edb_solver* s) {
   EDB_TRY(fft_prepare(s->p.N));
+  if (s->p.conserve) EDB_TRY(ensure_moment_table(s->qc));
   if (batched(s)) {
     const int groups = (s->p.batch + 63) / 64;
     PrefixEntry* pe = nullptr;

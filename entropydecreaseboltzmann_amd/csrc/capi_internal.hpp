@@ -68,6 +68,10 @@ struct edb_qcalc {
   double* d_slots = nullptr;
   size_t slots_count = 0;
   int nwaves = 0;
+  // conservative moment correction
+  int conserve = 0;
+  double* d_mom = nullptr;
+  int mom_E = 0;
   std::mutex mu;
 };
 
@@ -76,6 +80,8 @@ namespace edb {
 int ensure_tables(edb_qcalc* q);
 int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out);
 int ensure_slots(edb_qcalc* q, int nwaves);
+// Device table of the moment correction for q's (N, L), built on first use.
+int ensure_moment_table(edb_qcalc* q);
 // Batched Q on device arrays already in the kernel layout (float2 [M^3][ld] in,
 // double2 [M^3][ld] out); events, if given, bracket the main kernel.
 int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, const float* d_fin,

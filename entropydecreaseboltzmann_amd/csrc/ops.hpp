@@ -84,6 +84,12 @@ int launch_bkw_error(int N, int batch, int ld, double L, double K, const double*
 // calq_batched.hip
 int batched_ksplit(int N);
 size_t batched_slot_doubles(int N, int nwaves);
+
+// Conservative moment correction (conserve.hip): table of the 3M-2 axis
+// modes (centred index and the five moment transforms) followed by the 5x5
+// inverse moment matrix; the correction of Q [M^3][ld] centred, in place.
+int build_moment_table(int N, double L, std::vector<double>* tab, int* E);
+int launch_moment_correct(const double* d_tab, int E, int batch, int ld, double* d_q, hipStream_t s);
 bool batched_supported(int N);
 int build_pair_plan(int N, std::vector<int4>* rowinfo, std::vector<uint32_t>* sched, int* nrows_half);
 int batched_prefix(const std::vector<int4>& rowinfo, int nrows, int groups, int ks,

@@ -146,6 +146,7 @@ int make_group(edb_server* s, int N, int alpha, Group** out) {
   if (st == kOk) st = dmalloc(&g->steps_done, (size_t)g->cap);
   if (st == kOk) st = dmalloc(&g->newflag, (size_t)g->cap);
   if (st == kOk) st = fft_prepare(N);
+  if (st == kOk && s->p.conserve) st = ensure_moment_table(g->qc);
   if (st == kOk) {
     PrefixEntry* pe = nullptr;
     for (int grp = 1; grp <= g->cap / 64 && st == kOk; ++grp) st = get_prefix(g->qc, grp, 1, &pe);
@@ -207,7 +208,9 @@ int rk4_step(Group* g, const edb_server_params& p) {
   const double dt = p.dt;
   auto qe = [&](const double* X, const double* Qp, double c0, double* Qo) -> int {
     EDB_TRY(launch_pack(N, b, ld, true, g->in, X, Qp, c0, g->stream));
-    return batched_device(g->qc, b, ld, 1, Qo, g->in, g->stream);
+    EDB_TRY(batched_device(g->qc, b, ld, 1, Qo, g->in, g->stream));
+    if (p.conserve) EDB_TRY(launch_moment_correct(g->qc->d_mom, g->qc->mom_E, b, ld, Qo, g->stream));
+    return kOk;
   };
   EDB_TRY(qe(g->Fh, nullptr, 0., g->Q[0]));
   EDB_TRY(qe(g->Fh, g->Q[0], 0.5 * dt, g->Q[1]));

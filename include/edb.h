@@ -129,6 +129,14 @@ int edb_qcalc_tune(edb_qcalc* q, int batched_waves, int plain_slabs);
  * the fastest; returns them.  No reference counterpart. */
 int edb_qcalc_autotune(edb_qcalc* q, int batch, int* batched_waves, int* plain_slabs);
 
+/* Conservative moment correction of every Q this handle returns (calQ,
+ * calQ_device, calQ_batched, calQ_batched_device): the least-squares nearest
+ * spectrum on the Galerkin modes whose discrete mass, momentum and energy
+ * moments (h^3 sum_x {1, v, |v|^2} Q(x) on the grid) are zero.  Off by
+ * default.  No counterpart in the reference, whose Q conserves mass only;
+ * it is the constrained-L2 projection of conservative spectral methods. */
+int edb_qcalc_set_conservative(edb_qcalc* q, int on);
+
 int edb_qcalc_destroy(edb_qcalc* q);
 
 /* ------------------------------------------------------ fft3d (L2) ----- */
@@ -183,6 +191,7 @@ typedef struct {
   int fix_mode;    /* EDB_FIX_* */
   int hermitian;   /* batched kernel evaluates half the k rows (real data) */
   int history;     /* steps of entropy history kept on the device (0: none) */
+  int conserve;    /* moment correction of every Q (edb_qcalc_set_conservative) */
 } edb_solver_params;
 
 typedef struct edb_solver edb_solver;
@@ -243,6 +252,7 @@ typedef struct {
   int max_slots;     /* slots per group (0: 256, 64 for N >= 27) */
   int max_chunk;     /* RK4 steps per launch round (0: 16) */
   long hbm_budget;   /* bytes for kernels + slots over all groups (0: unlimited) */
+  int conserve;      /* moment correction of every Q (edb_qcalc_set_conservative) */
 } edb_server_params;
 
 typedef struct edb_server edb_server;

@@ -60,6 +60,9 @@ class QCalculator {
                                  reinterpret_cast<const double*>(fin)),
           "calQ_batched");
   }
+  // Moment correction of every Q (zero discrete mass, momentum and energy
+  // moments); no reference counterpart.
+  void setConservative(bool on = true) { check(edb_qcalc_set_conservative(h_, on ? 1 : 0), "setConservative"); }
   edb_qcalc* handle() { return h_; }
   int size() const { return N_; }
 
@@ -118,6 +121,7 @@ class BKW3D {
   void setFixApplied(bool on = true) { applied_ = on; }
   void setInitial(std::function<r_type(r_type, r_type, r_type, r_type)> f) { init_ = f; }
   void setPrecision(int precision) { precision_ = precision; }
+  void setConservative(bool on = true) { conserve_ = on; }
 
   void FEuler(r_type T0, r_type TEnd, r_type dt) { run(EDB_SCHEME_FEULER, T0, TEnd, dt, 0.); }
   void RK2(r_type T0, r_type TEnd, r_type dt) { run(EDB_SCHEME_RK2, T0, TEnd, dt, 0.); }
@@ -151,6 +155,7 @@ class BKW3D {
     p.beta = beta;
     p.fix_mode = fix_ ? (applied_ ? EDB_FIX_APPLIED : EDB_FIX_DIAGNOSTIC) : EDB_FIX_NONE;
     p.hermitian = 1;
+    p.conserve = conserve_ ? 1 : 0;
     double est = (TEnd - T0) / dt;
     p.history = (int)(est > 0 ? (est < 1e6 ? est : 1e6) : 0) + 16;
     edb_solver* s = nullptr;
@@ -185,7 +190,7 @@ class BKW3D {
   r_type R_;
   int alpha_;
   bool filter_;
-  bool fix_ = false, applied_ = false;
+  bool fix_ = false, applied_ = false, conserve_ = false;
   int precision_ = EDB_PREC_FP64_PLAIN;
   std::function<r_type(r_type, r_type, r_type, r_type)> init_;
   std::vector<r_type> F_, H_;

@@ -71,6 +71,9 @@ def lib():
         o.edbo_grid_bkw.argtypes = [I, D, D, _dp]
         o.edbo_run.restype = I
         o.edbo_run.argtypes = [I, _dp, D, I, D, D, D, D, I, _dp, I, _dp, _dp, _dp, I]
+        o.edbo_run_ex.restype = I
+        o.edbo_run_ex.argtypes = [I, _dp, D, I, D, D, D, D, I, I, _dp, I, _dp, _dp, _dp, I]
+        o.edbo_moment_correct.argtypes = [I, D, _dp]
         _o = o
     return _o
 
@@ -155,15 +158,24 @@ def grid_bkw(N, R, K):
 SCHEMES = {"FEuler": 0, "RK2": 1, "SSP3": 2, "RK4": 3, "CFE": 4, "DUFORT": 5}
 
 
-def run(N, B, R, scheme, T0, TEnd, dt, fhat, beta=0.0, fix_mode=0, max_steps=100000, nthreads=8):
+def moment_correct(N, L, qhat):
+    """Conservative moment correction of a spectrum (FFTW order); returns a copy."""
+    q = np.array(qhat, dtype=np.complex128, copy=True).reshape(N, N, N)
+    lib().edbo_moment_correct(N, L, _p(q.view(np.float64)))
+    return q
+
+
+def run(N, B, R, scheme, T0, TEnd, dt, fhat, beta=0.0, fix_mode=0, max_steps=100000, nthreads=8,
+        conserve=False):
     """BKW3D driver restated on a spectral state; returns (fhat, t, H, Hfix)."""
     f = np.array(fhat, dtype=np.complex128, copy=True).reshape(N, N, N)
     t = np.zeros(max_steps)
     H = np.zeros(max_steps)
     Hf = np.zeros(max_steps)
     sc = SCHEMES[scheme] if isinstance(scheme, str) else scheme
-    n = lib().edbo_run(N, _p(np.ascontiguousarray(B)), R, sc, T0, TEnd, dt, beta, fix_mode,
-                       _p(f.view(np.float64)), max_steps, _p(t), _p(H), _p(Hf), nthreads)
+    n = lib().edbo_run_ex(N, _p(np.ascontiguousarray(B)), R, sc, T0, TEnd, dt, beta, fix_mode,
+                          int(bool(conserve)), _p(f.view(np.float64)), max_steps, _p(t), _p(H),
+                          _p(Hf), nthreads)
     k = min(n, max_steps)
     return f, t[:k], H[:k], Hf[:k]
 

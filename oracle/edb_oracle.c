@@ -343,6 +343,105 @@ void edbo_ifft(int N, double* re, const double* fre) {
   for (long k = 0; k < 2 * N3; ++k) re[k] *= s;
 }
 
+/* ------------------------------------------ conservative correction -- */
+
+/* Conservative moment correction of a collision operator's spectrum.  No
+ * counterpart in the reference (its Q conserves mass only, through
+ * Q_0 == 0).  The discrete moments m_i(g) = h^3 sum_x phi_i(x) g(x) over the
+ * grid, phi in {1, v1, v2, v3, |v|^2}, are linear functionals of the
+ * spectrum: with P_i = DFT(phi_i) (forward, unnormalised),
+ *   m_i(g) = h^3/N^3 sum_k ghat_k conj(P_i,k).
+ * The corrected spectrum is the least-squares nearest one (discrete L2 norm
+ * over the Galerkin modes G) with all five moments zero:
+ *   Qhat' = Qhat - sum_j mu_j P_j (on G),   A mu = m,   A_ij = sum_{k in G} P_j,k conj(P_i,k).
+ * For odd N (G = every mode) this is the constrained L2 projection of the
+ * grid values, Q' = Q - C^T (C C^T)^-1 C Q.  Restated here with P computed by
+ * the O(N^4) DFT of the five grid functions; Qhat (FFTW order) in place. */
+static double* mc_P = NULL; /* P of the last (N, L), kept between calls */
+static int mc_N = 0;
+static double mc_L = 0.;
+
+void edbo_moment_correct(int N, double L, double* Qhat) {
+  long N3 = (long)N * N * N;
+  double h = 2. * L / (double)N;
+  double* g = (double*)malloc(sizeof(double) * 2 * N3);
+  if (mc_P && (mc_N != N || mc_L != L)) {
+    free(mc_P);
+    mc_P = NULL;
+  }
+  int fresh = mc_P == NULL;
+  if (fresh) mc_P = (double*)malloc(sizeof(double) * 2 * N3 * 5);
+  double* P = mc_P;
+  mc_N = N;
+  mc_L = L;
+  for (int i = 0; i < 5 && fresh; ++i) {
+    for (long x = 0; x < N3; ++x) {
+      int j1 = (int)(x / ((long)N * N)), j2 = (int)((x / N) % N), j3 = (int)(x % N);
+      double v1 = -L + j1 * h, v2 = -L + j2 * h, v3 = -L + j3 * h;
+      double phi = i == 0 ? 1. : i == 1 ? v1 : i == 2 ? v2 : i == 3 ? v3 : v1 * v1 + v2 * v2 + v3 * v3;
+      g[2 * x] = phi;
+      g[2 * x + 1] = 0.;
+    }
+    edbo_fft(N, g, P + 2 * N3 * i);
+    if (N % 2 == 0) /* Nyquist planes are outside G */
+      for (long x = 0; x < N3; ++x) {
+        int j1 = (int)(x / ((long)N * N)), j2 = (int)((x / N) % N), j3 = (int)(x % N);
+        if (2 * j1 == N || 2 * j2 == N || 2 * j3 == N) P[2 * (N3 * i + x)] = P[2 * (N3 * i + x) + 1] = 0.;
+      }
+  }
+  /* A (real: G is symmetric under k -> -k and P_i,-k = conj(P_i,k)), m complex */
+  double A[5][5], mr[5], mi[5];
+  for (int i = 0; i < 5; ++i) {
+    mr[i] = mi[i] = 0.;
+    for (long x = 0; x < N3; ++x) {
+      double pr = P[2 * (N3 * i + x)], pi = -P[2 * (N3 * i + x) + 1]; /* conj(P_i) */
+      double qr = Qhat[2 * x], qi = Qhat[2 * x + 1];
+      mr[i] += qr * pr - qi * pi;
+      mi[i] += qr * pi + qi * pr;
+    }
+    for (int j = 0; j < 5; ++j) {
+      double a = 0.;
+      for (long x = 0; x < N3; ++x)
+        a += P[2 * (N3 * j + x)] * P[2 * (N3 * i + x)] + P[2 * (N3 * j + x) + 1] * P[2 * (N3 * i + x) + 1];
+      A[i][j] = a;
+    }
+  }
+  /* Gaussian elimination with partial pivoting on the diagonally scaled
+   * system (the functionals differ by powers of L). */
+  double d[5], M[5][7];
+  for (int i = 0; i < 5; ++i) d[i] = 1. / sqrt(A[i][i]);
+  for (int i = 0; i < 5; ++i) {
+    for (int j = 0; j < 5; ++j) M[i][j] = d[i] * A[i][j] * d[j];
+    M[i][5] = d[i] * mr[i];
+    M[i][6] = d[i] * mi[i];
+  }
+  for (int c = 0; c < 5; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 5; ++r)
+      if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
+    for (int j = 0; j < 7; ++j) {
+      double t = M[c][j]; M[c][j] = M[piv][j]; M[piv][j] = t;
+    }
+    for (int r = 0; r < 5; ++r) {
+      if (r == c) continue;
+      double f = M[r][c] / M[c][c];
+      for (int j = c; j < 7; ++j) M[r][j] -= f * M[c][j];
+    }
+  }
+  double mur[5], mui[5];
+  for (int i = 0; i < 5; ++i) {
+    mur[i] = d[i] * M[i][5] / M[i][i];
+    mui[i] = d[i] * M[i][6] / M[i][i];
+  }
+  for (long x = 0; x < N3; ++x)
+    for (int j = 0; j < 5; ++j) {
+      double pr = P[2 * (N3 * j + x)], pi = P[2 * (N3 * j + x) + 1];
+      Qhat[2 * x] -= mur[j] * pr - mui[j] * pi;
+      Qhat[2 * x + 1] -= mur[j] * pi + mui[j] * pr;
+    }
+  free(g);
+}
+
 /* ---------------------------------------------------------- EntropyFix -- */
 
 /* EntropyFix(Nt, eps), EntropyFix.h:29-32; Averagef = 1.0 until set, :20. */
@@ -487,9 +586,23 @@ static double post_step(post_ctx* p, double t, int eligible, double entropy0) {
   return entropy;
 }
 
+/* calQ of the driver, followed by the conservative correction when asked. */
+static void drv_calQ(int N, const double* B, double R, int conserve, const double* F, double* Q,
+                     int nthreads) {
+  edbo_calQ(N, B, F, Q, nthreads);
+  if (conserve) edbo_moment_correct(N, R, Q);
+}
+
 int edbo_run(int N, const double* B, double R, int scheme, double T0, double TEnd, double dt,
              double beta_cfe, int fix_mode, double* Fhat, int max_steps, double* t_out,
              double* H_out, double* Hfix_out, int nthreads) {
+  return edbo_run_ex(N, B, R, scheme, T0, TEnd, dt, beta_cfe, fix_mode, 0, Fhat, max_steps, t_out,
+                     H_out, Hfix_out, nthreads);
+}
+
+int edbo_run_ex(int N, const double* B, double R, int scheme, double T0, double TEnd, double dt,
+                double beta_cfe, int fix_mode, int conserve, double* Fhat, int max_steps,
+                double* t_out, double* H_out, double* Hfix_out, int nthreads) {
   long N3 = (long)N * N * N, n2 = 2 * N3;
   size_t cb = sizeof(double) * 2 * N3;
   double *Q = malloc(cb), *Q1 = malloc(cb), *Q2 = malloc(cb), *Q3 = malloc(cb);
@@ -510,7 +623,7 @@ int edbo_run(int N, const double* B, double R, int scheme, double T0, double TEn
     step = 1;
     do {
       t += dt;
-      edbo_calQ(N, B, Fhat, Q, nthreads);
+      drv_calQ(N, B, R, conserve, Fhat, Q, nthreads);
       for (long l = 0; l < n2; ++l) Fhat[l] = Fhat[l] + dt * Q[l];
       entropy0 = post_step(&p, t, step > 1, entropy0);
       step++;
@@ -519,7 +632,7 @@ int edbo_run(int N, const double* B, double R, int scheme, double T0, double TEn
     double s = (1. - exp(-beta_cfe * dt)) / beta_cfe;
     do {
       t += dt;
-      edbo_calQ(N, B, Fhat, Q, nthreads);
+      drv_calQ(N, B, R, conserve, Fhat, Q, nthreads);
       for (long l = 0; l < n2; ++l) Fhat[l] = Fhat[l] + s * Q[l];
       entropy0 = post_step(&p, t, step > 0, entropy0);
       step++;
@@ -529,9 +642,9 @@ int edbo_run(int N, const double* B, double R, int scheme, double T0, double TEn
     double tstep = scheme == 5 ? dt / 8 : dt;
     do {
       t += tstep;
-      edbo_calQ(N, B, Fhat, Q, nthreads);
+      drv_calQ(N, B, R, conserve, Fhat, Q, nthreads);
       for (long l = 0; l < n2; ++l) F1[l] = Fhat[l] + 0.5 * dt * Q[l];
-      edbo_calQ(N, B, F1, Q, nthreads);
+      drv_calQ(N, B, R, conserve, F1, Q, nthreads);
       for (long l = 0; l < n2; ++l) Fhat[l] = Fhat[l] + dt * Q[l];
       entropy0 = post_step(&p, t, fixable && step > 0, entropy0);
       step++;
@@ -540,7 +653,7 @@ int edbo_run(int N, const double* B, double R, int scheme, double T0, double TEn
       step = 0;
       do {
         t += dt;
-        edbo_calQ(N, B, Fhat, Q, nthreads);
+        drv_calQ(N, B, R, conserve, Fhat, Q, nthreads);
         memcpy(F1, Fhat, cb);
         for (long l = 0; l < n2; ++l)
           Fhat[l] = 1. / (1. + beta_cfe * dt) * (F1[l] + dt * Q[l] + beta_cfe * dt * (F1[l] - Fold[l]));
@@ -552,11 +665,11 @@ int edbo_run(int N, const double* B, double R, int scheme, double T0, double TEn
   } else if (scheme == 2) { /* SSP3, BKW.h:413-446 */
     do {
       t += dt;
-      edbo_calQ(N, B, Fhat, Q, nthreads);
+      drv_calQ(N, B, R, conserve, Fhat, Q, nthreads);
       for (long l = 0; l < n2; ++l) F1[l] = Fhat[l] + dt * Q[l];
-      edbo_calQ(N, B, F1, Q, nthreads);
+      drv_calQ(N, B, R, conserve, F1, Q, nthreads);
       for (long l = 0; l < n2; ++l) F2[l] = 0.75 * Fhat[l] + 0.25 * F1[l] + 0.25 * dt * Q[l];
-      edbo_calQ(N, B, F2, Q, nthreads);
+      drv_calQ(N, B, R, conserve, F2, Q, nthreads);
       for (long l = 0; l < n2; ++l) Fhat[l] = Fhat[l] / 3. + 2. * F2[l] / 3. + 2. / 3. * dt * Q[l];
       entropy0 = post_step(&p, t, step > 0, entropy0);
       step++;
@@ -564,13 +677,13 @@ int edbo_run(int N, const double* B, double R, int scheme, double T0, double TEn
   } else { /* RK4, BKW.h:478-516 */
     do {
       t += dt;
-      edbo_calQ(N, B, Fhat, Q, nthreads);
+      drv_calQ(N, B, R, conserve, Fhat, Q, nthreads);
       for (long l = 0; l < n2; ++l) F1[l] = Fhat[l] + 0.5 * dt * Q[l];
-      edbo_calQ(N, B, F1, Q1, nthreads);
+      drv_calQ(N, B, R, conserve, F1, Q1, nthreads);
       for (long l = 0; l < n2; ++l) F2[l] = Fhat[l] + 0.5 * dt * Q1[l];
-      edbo_calQ(N, B, F2, Q2, nthreads);
+      drv_calQ(N, B, R, conserve, F2, Q2, nthreads);
       for (long l = 0; l < n2; ++l) F3[l] = Fhat[l] + dt * Q2[l];
-      edbo_calQ(N, B, F3, Q3, nthreads);
+      drv_calQ(N, B, R, conserve, F3, Q3, nthreads);
       for (long l = 0; l < n2; ++l)
         Fhat[l] = Fhat[l] + dt / 6. * (Q[l] + 2. * Q1[l] + 2. * Q2[l] + Q3[l]);
       entropy0 = post_step(&p, t, step > 0, entropy0);

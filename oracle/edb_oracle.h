@@ -85,6 +85,17 @@ int    edbo_run(int N, const double* B, double R, int scheme, double T0,
                 double TEnd, double dt, double beta_cfe, int fix_mode,
                 double* Fhat, int max_steps, double* t_out,
                 double* H_out, double* Hfix_out, int nthreads);
+/* edbo_run with every Q passed through edbo_moment_correct when conserve. */
+int    edbo_run_ex(int N, const double* B, double R, int scheme, double T0,
+                   double TEnd, double dt, double beta_cfe, int fix_mode,
+                   int conserve, double* Fhat, int max_steps, double* t_out,
+                   double* H_out, double* Hfix_out, int nthreads);
+
+/* Conservative moment correction (no counterpart in the reference): the
+ * least-squares nearest spectrum on the Galerkin modes whose discrete mass,
+ * momentum and energy moments vanish.  Qhat N^3 complex, FFTW order, in
+ * place; L the velocity half-width. */
+void   edbo_moment_correct(int N, double L, double* Qhat);
 
 #ifdef __cplusplus
 }

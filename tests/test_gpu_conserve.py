@@ -1,0 +1,125 @@
+"""Conservative moment correction on the GPU (edb_qcalc_set_conservative,
+edb_solver_params.conserve, edb_server_params.conserve) against the oracle's
+correction, and conservation of mass, momentum and energy to 1e-12 relative
+along FP64 and FP32 batched RK4 runs."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import maxwellian_mixture, rand_spectrum
+from test_oracle_conserve import moment_scale, moments
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def axis_mask(N):
+    m = np.zeros((N, N, N), dtype=bool)
+    m[:, 0, 0] = m[0, :, 0] = m[0, 0, :] = True
+    return m
+
+
+@pytest.mark.parametrize("N,alpha", [(9, 0), (8, 1), (16, 0)])
+def test_plain_corrected_matches_oracle(edb, N, alpha):
+    L = 8.0
+    f = np.fft.fftn(maxwellian_mixture(N, L, 60 + N))
+    q = edb.QCalculator(N, alpha, L)
+    q.closeFilter()
+    q.genB()
+    q.set_conservative(True)
+    got = q.calQ(f)
+    B = oracle.genB(N, alpha, L, 0)
+    ref = oracle.moment_correct(N, L, oracle.calQ(N, B, f))
+    assert rel(got, ref) < 1e-12
+    assert np.abs(moments(N, L, got)).max() <= 1e-13 * moment_scale(N, L, got)
+    q.set_conservative(False)
+    assert rel(q.calQ(f), oracle.calQ(N, B, f)) < 1e-12
+
+
+@pytest.mark.parametrize("N,batch,hermitian", [(9, 3, True), (16, 65, True), (24, 2, True),
+                                               (8, 2, False)])
+def test_batched_corrected_matches_oracle(edb, N, batch, hermitian):
+    L = 8.0
+    if hermitian:
+        fs = np.stack([np.fft.fftn(maxwellian_mixture(N, L, 80 + i)) for i in range(batch)])
+    else:
+        fs = np.stack([rand_spectrum(N, 90 + i) for i in range(batch)])
+    q = edb.QCalculator(N, 0, L)
+    q.closeFilter()
+    q.genB_batched()
+    q.set_conservative(True)
+    got = q.calQ_batched(fs, hermitian=hermitian)
+    B = oracle.genB(N, 0, L, 0)
+    ax = axis_mask(N)
+    for i in sorted({0, batch - 1}):
+        ref = oracle.moment_correct(N, L, oracle.calQ(N, B, fs[i]))
+        S = np.abs(oracle.calQ(N, np.abs(B), np.abs(fs[i]).astype(np.complex128)))
+        # FP32 errors of the kernel (bound S); the correction redistributes
+        # those of the axis modes along the axes
+        tol = 3e-6 * S + np.where(ax, 3e-6 * S[ax].sum(), 0.0) + 1e-300
+        assert np.all(np.abs(got[i] - ref) <= tol), (N, i)
+        # the device result itself conserves, whatever its FP32 error
+        assert np.abs(moments(N, L, got[i])).max() <= 1e-13 * moment_scale(N, L, got[i])
+
+
+def test_solver_plain_rk4_corrected_matches_oracle(edb):
+    N, L, dt = 9, 8.0, 0.05
+    F0 = maxwellian_mixture(N, L, 11)
+    s = edb.Solver(N, 0, L, False, 1, edb.PREC_FP64_PLAIN, "RK4", dt, 0.0, edb.FIX_NONE,
+                   history=16, conserve=True)
+    s.set_initial_real(F0[None])
+    s.run(0.5 - 1e-12)
+    B = oracle.genB(N, 0, L, 0)
+    fo, *_ = oracle.run(N, B, L, "RK4", 0.0, 0.5 - 1e-12, dt, np.fft.fftn(F0), fix_mode=0,
+                        conserve=True)
+    assert rel(s.get_spectral()[0], fo) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["plain", "batched"])
+def test_solver_conserves_mass_momentum_energy(edb, precision):
+    N, L, dt = 16, 8.0, 0.01
+    batch = 3 if precision == "plain" else 70
+    prec = edb.PREC_FP64_PLAIN if precision == "plain" else edb.PREC_FP32_BATCHED
+    F0 = np.stack([maxwellian_mixture(N, L, 200 + i) for i in range(batch)])
+    drift = {}
+    for conserve in (False, True):
+        s = edb.Solver(N, 0, L, False, batch, prec, "RK4", dt, 0.0, edb.FIX_NONE,
+                       hermitian=True, conserve=conserve)
+        s.set_initial_real(F0)
+        m0 = s.moments()
+        s.step(20)
+        m1 = s.moments()
+        scale = np.abs(m0[:, [0, 0, 0, 0, 4]]).max(axis=1, keepdims=True)
+        drift[conserve] = np.abs(m1 - m0) / scale
+    assert drift[True].max() <= 1e-12, drift[True].max(axis=0)
+    assert drift[False][:, 4].max() > 1e2 * drift[True][:, 4].max()
+
+
+def test_server_conserve_equals_solo(edb):
+    L, dt = 8.0, 0.01
+    srv = edb.Server(L=L, dt=dt, fix_mode=edb.FIX_NONE, max_slots=64, max_chunk=4, conserve=True)
+    F0s = [maxwellian_mixture(16, L, 300 + i) for i in range(5)]
+    for i, F0 in enumerate(F0s):
+        assert srv.submit(i, 16, 0, 0.05 + 0.01 * i, F0) == 0
+    out = {}
+    for _ in range(100000):
+        if srv.poll():
+            while True:
+                try:
+                    rid, N, steps, t, F = srv.fetch()
+                except Exception:
+                    break
+                out[rid] = F
+        if len(out) == 5:
+            break
+    assert len(out) == 5
+    for i in (0, 4):
+        s = edb.Solver(16, 0, L, False, 1, edb.PREC_FP32_BATCHED, "RK4", dt, 0.0, edb.FIX_NONE,
+                       hermitian=True, conserve=True)
+        s.set_initial_real(F0s[i][None])
+        s.run(0.05 + 0.01 * i)
+        Fa = s.get_real()[0]
+        assert np.abs(out[i] - Fa).max() <= 1e-12 * np.abs(Fa).max(), i
