@@ -216,6 +216,251 @@ def serve(edb, args):
     print(json.dumps(out), flush=True)
 
 
+# ------------------------------------------------------------------ experiments
+# The reference's own configurations, run end to end: errors against the
+# exact BKW solution and against the FP64 CPU oracle (used here only as the
+# checker), conservation and the H-theorem.  One JSON line each.
+
+def bkw_grid(N, L, K):
+    """BKW profile f(K, v) on x_i = -L + i*2L/N (testfft.cpp:11-16 form)."""
+    h = 2 * L / N
+    x = -L + h * np.arange(N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    v2 = X * X + Y * Y + Z * Z
+    return (2 * np.pi * K) ** -1.5 * np.exp(-v2 / (2 * K)) * (
+        (5 * K - 3) / (2 * K) + (1 - K) / (2 * K * K) * v2)
+
+
+def driver_steps(T0, TEnd, dt):
+    """Step count of the drivers' do { t += dt } while (t < TEnd) loop."""
+    t, n = T0, 0
+    while True:
+        t += dt
+        n += 1
+        if not t < TEnd:
+            return n
+
+
+def err_stats(F, ref, h3):
+    d = np.abs(F - ref)
+    return {"linf": float(d.max()), "l1": float(d.sum() * h3),
+            "linf_rel": float(d.max() / np.abs(ref).max())}
+
+
+def cpu_chunked(N, L, alpha, F0, dt, n, chunk, progress, K_of=None):
+    """FP64 CPU oracle RK4 (no fix) in chunks of steps; BKW errors per chunk."""
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    B = oracle.genB(N, alpha, L, False)
+    fh = np.fft.fftn(F0)
+    h3 = (2 * L / N) ** 3
+    done, worst, t0 = 0, 0.0, time.perf_counter()
+    while done < n:
+        c = min(chunk, n - done)
+        T0 = done * dt
+        fh, *_ = oracle.run(N, B, L, "RK4", T0, T0 + (c - 0.5) * dt, dt, fh, fix_mode=0,
+                            nthreads=threads)
+        done += c
+        if K_of is not None:
+            F = np.fft.ifftn(fh).real
+            worst = max(worst, err_stats(F, bkw_grid(N, L, K_of(done * dt)), h3)["linf"])
+        progress(f"cpu oracle N={N}: {done}/{n} steps, {time.perf_counter() - t0:.0f} s")
+    return np.fft.ifftn(fh).real, worst, threads, time.perf_counter() - t0
+
+
+def gpu_bkw_run(edb, N, L, F0, dt, n, precision, fix_mode, K_of, conserve=False):
+    s = edb.Solver(N, 0, L, False, 1, precision, "RK4", dt, 0.0, fix_mode, hermitian=True,
+                   history=n + 8, conserve=conserve)
+    s.set_initial_real(F0[None])
+    s.enable_qtiming(True)
+    worst_linf, worst_l1 = 0.0, 0.0
+    t0 = time.perf_counter()
+    for k in range(1, n + 1):
+        s.step(1)
+        e = s.bkw_error(K_of(k * dt))[0]
+        worst_linf, worst_l1 = max(worst_linf, e[0]), max(worst_l1, e[1])
+    el = time.perf_counter() - t0
+    qms, nl = s.qkernel_time()
+    H, Hf, fixed = s.entropy_history(n)
+    F = s.get_real()[0]
+    m = s.moments()[0]
+    s.close()
+    return {"F": F, "max_linf_vs_bkw": float(worst_linf), "max_l1_vs_bkw": float(worst_l1),
+            "q_kernel_ms": qms, "launches": nl, "wall_s": el, "fix_steps": int(fixed.sum()),
+            "moments_end": m.tolist()}
+
+
+def experiment_bkw(edb, N, with_cpu_steps=None):
+    """N=16 (plain on CPU, fix off and on) or N=24 (plain FP64 beside FP32 beta
+    on the GPU): BKW initial data with K(0) = 0.5, RK4 dt = 0.01 to t = 5."""
+    L, dt, TEnd = 8.0, 0.01, 5.0
+    K_of = lambda t: 1.0 - 0.5 * np.exp(-t / 6.0)
+    n = driver_steps(0.0, TEnd, dt)
+    F0 = bkw_grid(N, L, K_of(0.0))
+    h3 = (2 * L / N) ** 3
+    prog = lambda m: print(m, file=sys.stderr, flush=True)
+    Fcpu, cpu_worst, threads, cpu_s = cpu_chunked(N, L, 0, F0, dt, n, 50, prog, K_of)
+    ref_end = bkw_grid(N, L, K_of(n * dt))
+    out = {"experiment": f"BKW N={N}", "N": N, "L": L, "alpha": 0, "filter": False, "dt": dt,
+           "t_end": round(n * dt, 10), "steps": n, "K0": 0.5,
+           "cpu_fp64": {"vs_bkw_end": err_stats(Fcpu, ref_end, h3), "max_linf_vs_bkw": cpu_worst,
+                        "threads": threads, "wall_s": round(cpu_s, 1)}}
+    runs = [("gpu_fp64_plain", edb.PREC_FP64_PLAIN, edb.FIX_NONE),
+            ("gpu_fp64_plain_fix_diagnostic", edb.PREC_FP64_PLAIN, edb.FIX_DIAGNOSTIC),
+            ("gpu_fp64_plain_fix_applied", edb.PREC_FP64_PLAIN, edb.FIX_APPLIED),
+            ("gpu_fp32_beta", edb.PREC_FP32_BATCHED, edb.FIX_NONE),
+            ("gpu_fp32_beta_fix_diagnostic", edb.PREC_FP32_BATCHED, edb.FIX_DIAGNOSTIC)]
+    q = edb.QCalculator(N, 0, L)
+    info = q.info()
+    q.close()
+    for name, prec, fix in runs:
+        r = gpu_bkw_run(edb, N, L, F0, dt, n, prec, fix, K_of)
+        beta_bytes = info["bytes_plain64"] if prec == edb.PREC_FP64_PLAIN else info["bytes_pair32"]
+        out[name] = {"vs_bkw_end": err_stats(r["F"], ref_end, h3),
+                     "max_linf_vs_bkw": r["max_linf_vs_bkw"],
+                     "vs_cpu_fp64_end": err_stats(r["F"], Fcpu, h3),
+                     "q_kernel_ms": round(r["q_kernel_ms"], 4),
+                     "beta_stream_gbs": round(beta_bytes / (r["q_kernel_ms"] * 1e-3) / 1e9, 1),
+                     "fix_steps": r["fix_steps"], "wall_s": round(r["wall_s"], 2)}
+        prog(f"{name}: done")
+    plain = out["gpu_fp64_plain"]["vs_bkw_end"]["linf"]
+    out["fp32_over_fp64_error_vs_bkw"] = round(out["gpu_fp32_beta"]["vs_bkw_end"]["linf"] / plain, 4)
+    return out
+
+
+def bimaxwellian_bump(N, L, seed):
+    """0.9 x bi-Maxwellian (T_par along z, T_perp; both U[0.3, 2]) + 0.1 x
+    Maxwellian (T = 0.5) centred at |u| ~ U[2, 3] in a random direction;
+    mass 1 on the grid."""
+    rng = np.random.default_rng(seed)
+    tpar, tperp = rng.uniform(0.3, 2.0, 2)
+    r = rng.uniform(2.0, 3.0)
+    d = rng.standard_normal(3)
+    u = r * d / np.linalg.norm(d)
+    h = 2 * L / N
+    x = -L + h * np.arange(N)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    bi = np.exp(-(X * X + Y * Y) / (2 * tperp) - Z * Z / (2 * tpar)) / (
+        (2 * np.pi) ** 1.5 * tperp * np.sqrt(tpar))
+    tb = 0.5
+    bump = np.exp(-((X - u[0]) ** 2 + (Y - u[1]) ** 2 + (Z - u[2]) ** 2) / (2 * tb)) / (2 * np.pi * tb) ** 1.5
+    f = 0.9 * bi / (bi.sum() * h ** 3) + 0.1 * bump / (bump.sum() * h ** 3)
+    return f
+
+
+def experiment_n32(edb, batch=64, steps=400, cpu_steps=4):
+    """64 instances at N=32, alpha=1, FP32 beta; RK4 dt=0.005, 400 steps;
+    FP64 conservation drift with and without the moment correction, and the
+    H-theorem (entropy non-increase) without the entropy fix."""
+    N, L, dt, alpha = 32, 8.0, 0.005, 1
+    prog = lambda m: print(m, file=sys.stderr, flush=True)
+    F0 = np.stack([bimaxwellian_bump(N, L, 3000 + i) for i in range(batch)])
+    q = edb.QCalculator(N, alpha, L)
+    q.closeFilter()
+    q.genB_batched()
+    q.autotune(batch)
+    out = {"experiment": "N=32 alpha=1 bi-Maxwellian + bump", "N": N, "alpha": alpha, "L": L,
+           "batch": batch, "dt": dt, "steps": steps, "beta_fp32_bytes": q.info()["bytes_pair32"]}
+    for conserve in (False, True):
+        s = edb.Solver(N, alpha, L, False, batch, edb.PREC_FP32_BATCHED, "RK4", dt, 0.0,
+                       edb.FIX_DIAGNOSTIC, hermitian=True, history=steps + 8, qcalc=q,
+                       conserve=conserve)
+        s.set_initial_real(F0)
+        m0 = s.moments()
+        s.enable_qtiming(True)
+        t0 = time.perf_counter()
+        s.step(steps)
+        edb.load().edb_sync()
+        el = time.perf_counter() - t0
+        m1 = s.moments()
+        qms, _ = s.qkernel_time()
+        H, _Hf, _fx = s.entropy_history(steps)
+        scale = np.abs(m0[:, [0, 0, 0, 0, 4]]).max(axis=1, keepdims=True)
+        drift = np.abs(m1 - m0) / scale
+        # calEntropy (BKW.h:140-161) is NaN once some f <= -1e-6; the H-theorem
+        # is checked on the instances whose entropy stays defined
+        finite = np.isfinite(H).all(axis=0)
+        Hd = H[:, finite]
+        dH = np.diff(Hd, axis=0)
+        rises = dH > 1e-12 * np.abs(Hd[:-1])
+        key = "conservative" if conserve else "plain_operator"
+        out[key] = {"drift_rel_max": {"mass": float(drift[:, 0].max()),
+                                      "momentum": float(drift[:, 1:4].max()),
+                                      "energy": float(drift[:, 4].max())},
+                    "H_undefined_instances": int((~finite).sum()),
+                    "H_first_nan_step_min": int(np.argmax(~np.isfinite(H), axis=0)[~finite].min())
+                    if (~finite).any() else None,
+                    "H_first_last_mean_defined": [float(Hd[0].mean()), float(Hd[-1].mean())]
+                    if Hd.size else None,
+                    "H_rises": int(rises.sum()), "H_rise_instances": int(rises.any(axis=0).sum()),
+                    "H_max_rise_rel": float(np.max(np.where(rises, dH / np.abs(Hd[:-1]), 0.0)))
+                    if Hd.size else None,
+                    "min_f_end": float(s.get_real().min()),
+                    "q_kernel_ms": round(qms, 4), "ms_per_step": round(el / steps * 1e3, 3),
+                    "qevals_per_s": round(4 * batch * steps / el, 1)}
+        if not conserve and cpu_steps:
+            # one sampled instance against the FP64 CPU oracle over the first steps
+            import oracle
+            threads = min(16, os.cpu_count() or 1)
+            s2 = edb.Solver(N, alpha, L, False, batch, edb.PREC_FP32_BATCHED, "RK4", dt, 0.0,
+                            edb.FIX_NONE, hermitian=True, qcalc=q)
+            s2.set_initial_real(F0)
+            s2.step(cpu_steps)
+            got = s2.get_real()[batch - 1]
+            B = oracle.genB(N, alpha, L, False)
+            prog(f"N=32 cpu check: genB done, {cpu_steps} RK4 steps on {threads} threads")
+            fo, *_ = oracle.run(N, B, L, "RK4", 0.0, (cpu_steps - 0.5) * dt, dt,
+                                np.fft.fftn(F0[batch - 1]), fix_mode=0, nthreads=threads)
+            ref = np.fft.ifftn(fo).real
+            out["cpu_fp64_check"] = {"instance": batch - 1, "steps": cpu_steps,
+                                     **err_stats(got, ref, (2 * L / N) ** 3)}
+            s2.close()
+        s.close()
+        prog(f"N=32 {key}: done")
+    return out
+
+
+def experiment_check256(edb, N=24, batch=256, steps=200, sample=8):
+    """The bench workload (256 instances, N=24, RK4 with the applied fix, 200
+    steps from one HIP graph), with 8 sampled instances run again by the FP64
+    CPU oracle."""
+    import oracle
+    L, dt = 8.0, 0.01
+    prog = lambda m: print(m, file=sys.stderr, flush=True)
+    F0 = np.stack([mixture(N, L, 1000 + i) for i in range(batch)])
+    q = edb.QCalculator(N, 0, L)
+    q.closeFilter()
+    q.genB_batched()
+    q.autotune(batch)
+    s = edb.Solver(N, 0, L, False, batch, edb.PREC_FP32_BATCHED, "RK4", dt, 0.0, edb.FIX_APPLIED,
+                   hermitian=True, history=steps + 8, qcalc=q)
+    s.set_initial_real(F0)
+    s.use_graph(True)
+    s.step(steps)
+    got = s.get_real()
+    _H, _Hf, fixed = s.entropy_history(steps)
+    threads = min(16, os.cpu_count() or 1)
+    B = oracle.genB(N, 0, L, False)
+    h3 = (2 * L / N) ** 3
+    idx = np.linspace(0, batch - 1, sample).astype(int)
+    res = []
+    t0 = time.perf_counter()
+    for i in idx:
+        fo, _t, Hc, Hfc = oracle.run(N, B, L, "RK4", 0.0, (steps - 0.5) * dt, dt,
+                                     np.fft.fftn(F0[i]), fix_mode=2, nthreads=threads)
+        ref = np.fft.ifftn(fo).real
+        e = err_stats(got[i], ref, h3)
+        # calEntropy is NaN once some f <= -1e-6 (BKW.h:153-161); no fix runs on such steps
+        e.update({"instance": int(i), "gpu_fix_steps": int(fixed[:, i].sum()),
+                  "cpu_fix_steps": int(((Hfc != Hc) & np.isfinite(Hc)).sum()),
+                  "entropy_undefined_steps": int((~np.isfinite(Hc)).sum())})
+        res.append(e)
+        prog(f"check256: instance {i} done ({time.perf_counter() - t0:.0f} s): linf_rel {e['linf_rel']:.2e}")
+    return {"experiment": "batched N=24 x 256, 8 instances vs FP64 CPU", "N": N, "batch": batch,
+            "steps": steps, "graph": True, "fix": "applied", "cpu_threads": threads,
+            "max_linf_rel": max(r["linf_rel"] for r in res), "instances": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,6 +478,8 @@ def main():
     ap.add_argument("--drain-max", type=float, default=60.0,
                     help="seconds allowed after the arrivals stop (--serve)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--experiment", choices=["bkw16", "bkw24", "n32", "check256"],
+                    help="run one of the reference configurations with its error report")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="no solver: one 256 MiB device copy (the reference transfer for PMC "
                          "scaling), then 3 batched and 3 plain Q launches at --N/--batch")
@@ -249,6 +496,17 @@ def main():
         return
     if args.serve:
         serve(edb, args)
+        return
+    if args.experiment:
+        if args.experiment == "bkw16":
+            res = experiment_bkw(edb, 16)
+        elif args.experiment == "bkw24":
+            res = experiment_bkw(edb, 24)
+        elif args.experiment == "n32":
+            res = experiment_n32(edb)
+        else:
+            res = experiment_check256(edb)
+        print(json.dumps(res), flush=True)
         return
     N, L, dt, batch = args.N, 8.0, 0.01, args.batch
     M = edb.nmodes(N)

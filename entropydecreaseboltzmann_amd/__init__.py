@@ -269,6 +269,19 @@ class Solver:
               "entropy_history")
         return H, Hf, fx
 
+    def mean_history(self, steps):
+        m = np.empty((steps, self.batch))
+        mf = np.empty((steps, self.batch))
+        check(load().edb_solver_mean_history(self._h, int(steps), _ptr(m), _ptr(mf)), "mean_history")
+        return m, mf
+
+    def get_driver_real(self):
+        """The driver's real-space F after the last step (fixed values where the fix ran)."""
+        N = self.N
+        out = np.empty((self.batch, N, N, N), dtype=np.float64)
+        check(load().edb_solver_get_driver_real(self._h, _ptr(out)), "get_driver_real")
+        return out
+
     def moments(self):
         out = np.empty((self.batch, 5))
         check(load().edb_solver_moments(self._h, _ptr(out)), "moments")

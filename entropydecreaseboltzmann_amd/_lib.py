@@ -73,6 +73,8 @@ _SIGS = {
     "edb_qcalc_tune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "edb_qcalc_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
     "edb_qcalc_set_conservative": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "edb_solver_mean_history": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+    "edb_solver_get_driver_real": (ctypes.c_int, [_vp, _dp]),
     "edb_qcalc_destroy": (ctypes.c_int, [_vp]),
     "edb_fft3d_fft": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "edb_fft3d_ifft": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),

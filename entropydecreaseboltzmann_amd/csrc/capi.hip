@@ -598,7 +598,7 @@ struct edb_solver {
   void* in = nullptr;
   double *H0 = nullptr, *avg = nullptr;
   int *avg_flag = nullptr, *fixed = nullptr;
-  double *hist_H = nullptr, *hist_Hfix = nullptr;
+  double *hist_H = nullptr, *hist_Hfix = nullptr, *hist_mean = nullptr, *hist_mean_fix = nullptr;
   int* hist_fixed = nullptr;
   int* d_rec = nullptr;
   double *d_out = nullptr;
@@ -680,6 +680,9 @@ int post_step(edb_solver* s, int eligible) {
   a.hist_H = s->hist_H;
   a.hist_Hfix = s->hist_Hfix;
   a.hist_fixed = s->hist_fixed;
+  a.Fh = (const double2*)s->Fh;
+  a.hist_mean = s->hist_mean;
+  a.hist_mean_fix = s->hist_mean_fix;
   a.rec_ptr = s->d_rec;
   a.hist_cap = s->p.history;
   a.eligible = eligible;
@@ -909,6 +912,8 @@ int edb_solver_create(const edb_solver_params* p, edb_qcalc* shared, edb_solver*
     st = dmalloc(&s->hist_H, (size_t)p->history * s->ld);
     if (st == kOk) st = dmalloc(&s->hist_Hfix, (size_t)p->history * s->ld);
     if (st == kOk) st = dmalloc(&s->hist_fixed, (size_t)p->history * s->ld);
+    if (st == kOk) st = dmalloc(&s->hist_mean, (size_t)p->history * s->ld);
+    if (st == kOk) st = dmalloc(&s->hist_mean_fix, (size_t)p->history * s->ld);
   }
   if (st == kOk) st = launch_fill(s->S2, grid, 0., s->stream);
   if (st == kOk) st = launch_fill((double*)s->fixed, (s->ld + 1) / 2, 0., s->stream);
@@ -1079,6 +1084,44 @@ int edb_solver_entropy_history(edb_solver* s, int steps, double* H, double* Hfix
   return EDB_OK;
 }
 
+int edb_solver_mean_history(edb_solver* s, int steps, double* mean, double* mean_fix) {
+  EDB_REQUIRE(s, "null handle");
+  EDB_REQUIRE(steps >= 0 && steps <= s->p.history, "mean_history: steps exceeds the history kept");
+  EDB_HIP_TRY(hipStreamSynchronize(s->stream));
+  const int b = s->p.batch, ld = s->ld;
+  std::vector<double> m((size_t)steps * ld), mf((size_t)steps * ld);
+  if (steps > 0) {
+    EDB_HIP_TRY(hipMemcpy(m.data(), s->hist_mean, sizeof(double) * m.size(), hipMemcpyDeviceToHost));
+    EDB_HIP_TRY(hipMemcpy(mf.data(), s->hist_mean_fix, sizeof(double) * mf.size(), hipMemcpyDeviceToHost));
+  }
+  for (int r = 0; r < steps; ++r)
+    for (int i = 0; i < b; ++i) {
+      if (mean) mean[(size_t)r * b + i] = m[(size_t)r * ld + i];
+      if (mean_fix) mean_fix[(size_t)r * b + i] = mf[(size_t)r * ld + i];
+    }
+  return EDB_OK;
+}
+
+int edb_solver_get_driver_real(edb_solver* s, double* F) {
+  EDB_REQUIRE(s && F, "get_driver_real: null argument");
+  const long N3 = s->g.N3;
+  double* dh = nullptr;
+  EDB_TRY(dmalloc(&dh, (size_t)N3 * s->p.batch));
+  int st = kOk;
+  // before the first step the driver's F is the inverse transform of the state
+  if (s->taken == 0)
+    st = launch_fft3d(s->p.N, s->p.N, s->p.N, s->p.batch, s->ld, +1, 1. / (double)N3, s->Fh, s->Fr,
+                      s->S2, s->stream);
+  if (st == kOk) st = launch_complex_to_real_cols(N3, s->p.batch, s->ld, s->Fr, dh, s->stream);
+  if (st == kOk && hipStreamSynchronize(s->stream) != hipSuccess) st = kHipError;
+  if (st == kOk &&
+      hipMemcpy(F, dh, sizeof(double) * N3 * s->p.batch, hipMemcpyDeviceToHost) != hipSuccess)
+    st = kHipError;
+  if (st == kHipError) set_error("get_driver_real: HIP failure");
+  dfree(&dh);
+  return st;
+}
+
 int edb_solver_moments(edb_solver* s, double* out) {
   EDB_REQUIRE(s && out, "moments: null argument");
   EDB_TRY(refresh_real(s));
@@ -1121,6 +1164,8 @@ int edb_solver_destroy(edb_solver* s) {
   dfree(&s->hist_H);
   dfree(&s->hist_Hfix);
   dfree(&s->hist_fixed);
+  dfree(&s->hist_mean);
+  dfree(&s->hist_mean_fix);
   dfree((double**)&s->d_rec);
   dfree(&s->d_out);
   if (s->own_qc) edb_qcalc_destroy(s->qc);

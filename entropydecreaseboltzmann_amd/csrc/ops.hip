@@ -249,6 +249,9 @@ __global__ __launch_bounds__(512) void entropy_fix_kernel(EntropyFixArgs A, int 
   const double H = block_sum(part, sh) * A.vol_over_N3;
   const double H0 = A.H0[inst];
   double Hfix = H;
+  // mean of Re F: the k = 0 coefficient of the state over N^3
+  const double mean = A.Fh ? A.Fh[inst].x / (double)N3 : 0.;
+  double mean_fix = mean;
   int fixed = 0;
   const bool elig = A.eligible && (!A.steps_done || A.steps_done[inst] > 0);
   if (elig && A.fix_mode != 0 && H - H0 > 1.0e-10) {
@@ -265,12 +268,17 @@ __global__ __launch_bounds__(512) void entropy_fix_kernel(EntropyFixArgs A, int 
       }
     }
     fixsol_block(w, N3, H0 / A.vol, A.eps, avg, sh);
-    double s2 = 0.;
-    for (long g = t; g < N3; g += kRedThreads) s2 += flogf_dev(w[g]);
+    double s2 = 0., s3 = 0.;
+    for (long g = t; g < N3; g += kRedThreads) {
+      s2 += flogf_dev(w[g]);
+      s3 += w[g];
+    }
     Hfix = block_sum(s2, sh) * A.vol_over_N3;
+    mean_fix = block_sum(s3, sh) / (double)N3;
     fixed = 1;
-    if (A.fix_mode == 2)
-      for (long g = t; g < N3; g += kRedThreads) A.F[g * ld + inst] = make_double2(w[g], 0.);
+    // the driver's real-space F holds the fixed values in both modes; only
+    // mode 2 feeds them back into the spectral state
+    for (long g = t; g < N3; g += kRedThreads) A.F[g * ld + inst] = make_double2(w[g], 0.);
   }
   if (t == 0) {
     A.H0[inst] = Hfix;
@@ -280,6 +288,10 @@ __global__ __launch_bounds__(512) void entropy_fix_kernel(EntropyFixArgs A, int 
       A.hist_H[(long)rec * ld + inst] = H;
       A.hist_Hfix[(long)rec * ld + inst] = Hfix;
       A.hist_fixed[(long)rec * ld + inst] = fixed;
+      if (A.hist_mean) {
+        A.hist_mean[(long)rec * ld + inst] = mean;
+        A.hist_mean_fix[(long)rec * ld + inst] = mean_fix;
+      }
     }
   }
 }

@@ -29,6 +29,9 @@ struct EntropyFixArgs {
   double* hist_H = nullptr;  // [rec][ld] optional history
   double* hist_Hfix = nullptr;
   int* hist_fixed = nullptr;
+  const double2* Fh = nullptr;   // spectral state (FFTW order) for the mean of F
+  double* hist_mean = nullptr;   // [rec][ld] mean of Re F before / after the fix
+  double* hist_mean_fix = nullptr;
   int rec = 0;              // history row (used when rec_ptr is null)
   const int* rec_ptr = nullptr;  // device counter of the history row
   int hist_cap = 0;

@@ -231,6 +231,13 @@ int edb_solver_get_real(edb_solver* s, double* F);
 /* Entropy history: H after each step, after its fix, and whether a fix ran,
  * [steps][batch] for the first min(steps, history) steps. */
 int edb_solver_entropy_history(edb_solver* s, int steps, double* H, double* Hfix, int* fixed);
+/* Mean of Re F after each step, before and after its entropy fix (the
+ * drivers' "average" column, BKW.h:288-305, 357-378), [steps][batch]. */
+int edb_solver_mean_history(edb_solver* s, int steps, double* mean, double* mean_fix);
+/* The driver's real-space F after the last step (BKW3D::F after fft3(true)
+ * and the fix, BKW.h:485-512): the entropy fix's values where it ran, in the
+ * diagnostic mode too, whose fixed F never reaches the state.  [batch][N^3]. */
+int edb_solver_get_driver_real(edb_solver* s, double* F);
 /* FP64 moments of F: [batch][5] = mass, momentum x,y,z, energy (sum |v|^2 f). */
 int edb_solver_moments(edb_solver* s, double* out);
 /* L-infinity and L1 error of F against the BKW profile at K, [batch][2]. */

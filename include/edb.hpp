@@ -16,10 +16,15 @@
 // velocity half-width L that the reference hard-codes (QCalculator.h:176).
 #pragma once
 
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <ctime>
+#include <fstream>
 #include <functional>
+#include <iomanip>
+#include <ostream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -112,7 +117,14 @@ class EntropyFix {
 
 // BKW3D drivers on one instance.  Initial data: a user function of
 // (t, x, y, z) on x_i = -R + i*2R/N (BKW.h:121-138); by default the BKW
-// profile of testfft.cpp:11-16, s = 1 - 0.4 exp(-t/6).
+// profile of testfft.cpp:11-16, s = 1 - 0.4 exp(-t/6).  Entropy fix as in
+// the reference drivers: RK4 and SSP3 always run it (BKW.h:432, 499),
+// FEuler and CFE when setFixEntropy is on (:228, :294), RK2 and DUFORT never
+// (their fix is compiled out, :359, :604); it is diagnostic (F fixed, the
+// spectral state untouched) unless setFixApplied.  With setOutput the
+// drivers print the reference's progress lines and per-step tables
+// (BKW.h:212-633) and FEuler writes F.dat (:217, :243-246); timing lines
+// carry this run's own clock.
 template <class c_type, class r_type>
 class BKW3D {
  public:
@@ -122,6 +134,10 @@ class BKW3D {
   void setInitial(std::function<r_type(r_type, r_type, r_type, r_type)> f) { init_ = f; }
   void setPrecision(int precision) { precision_ = precision; }
   void setConservative(bool on = true) { conserve_ = on; }
+  void setOutput(std::ostream* os, bool writeF = true) {
+    out_ = os;
+    writeF_ = writeF;
+  }
 
   void FEuler(r_type T0, r_type TEnd, r_type dt) { run(EDB_SCHEME_FEULER, T0, TEnd, dt, 0.); }
   void RK2(r_type T0, r_type TEnd, r_type dt) { run(EDB_SCHEME_RK2, T0, TEnd, dt, 0.); }
@@ -130,9 +146,26 @@ class BKW3D {
   void CFE(r_type T0, r_type TEnd, r_type dt, r_type beta) { run(EDB_SCHEME_CFE, T0, TEnd, dt, beta); }
   void DUFORT(r_type T0, r_type TEnd, r_type dt, r_type beta) { run(EDB_SCHEME_DUFORT, T0, TEnd, dt, beta); }
 
-  // Real-space F after the last step (N^3 values).
+  // The driver's real-space F after the last step (N^3 values; fixed values
+  // where the last step's fix ran) and the entropy after each step's fix.
   const std::vector<r_type>& F() const { return F_; }
   const std::vector<r_type>& entropy() const { return H_; }
+
+  // BKW3D::outV1Direction (BKW.h:163-180): x, F(x, h/2, h/2) at the grid
+  // point (i, N/2, N/2), and the initial function at (0, x, h/2, h/2).
+  void outV1Direction(const std::string& filename) const {
+    std::ofstream w(filename);
+    if (!w) throw std::runtime_error("outV1Direction: cannot open " + filename);
+    const r_type h = 2. * R_ / r_type(N0_);
+    auto f = init_ ? init_ : std::function<r_type(r_type, r_type, r_type, r_type)>(bkw);
+    w << "# x\tf(t)\tf_0\n";
+    for (int i = 0; i < N0_; ++i) {
+      const r_type x = -R_ + i * h;
+      const long l = ((long)i * N0_ + N0_ / 2) * N0_ + N0_ / 2;
+      const r_type fx = F_.empty() ? r_type(0) : F_[l];
+      w << std::fixed << std::setprecision(8) << x << "\t" << fx << "\t" << f(0, x, h / 2, h / 2) << "\n";
+    }
+  }
 
  private:
   static r_type bkw(r_type t, r_type x, r_type y, r_type z) {
@@ -142,7 +175,64 @@ class BKW3D {
            ((5. * s - 3.) / (2. * s) + (1. - s) / (2. * s * s) * v2);
   }
 
+  // TimeWatch's report (TimeWatch.h:24-33) for a span of this run.
+  struct Watch {
+    std::ostream* os;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    std::clock_t c0 = std::clock();
+    ~Watch() {
+      if (!os) return;
+      double w = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      double c = double(std::clock() - c0) / CLOCKS_PER_SEC;
+      *os << std::fixed << std::setprecision(2) << "#\tWTime: " << w << "s\t"
+          << "CTime: " << c << "s\n"
+          << "#\tSpeedup: " << (w > 0 ? c / w : 0.) << "\n";
+    }
+  };
+
+  // One table row of the reference's drivers for step k (0-based) of the
+  // current phase; prev is the entropy after the previous step's fix.
+  void row(int scheme, int k, int step_no, r_type t, r_type H, r_type Hf, bool fixed, r_type mean,
+           r_type mean_fix, r_type prev) {
+    std::ostream& o = *out_;
+    o << std::fixed << std::setprecision(8);
+    switch (scheme) {
+      case EDB_SCHEME_FEULER:  // BKW.h:226-241
+        if (step_no > 1)
+          o << step_no << "\t" << t << "\t" << H << "\t" << prev - H << "\t" << Hf << "\t" << prev - Hf
+            << std::endl;
+        else
+          o << step_no << "\t" << t << "\t" << H << "\t-\t-\t-\t\n";
+        break;
+      case EDB_SCHEME_CFE:  // BKW.h:293-309
+        if (k) {
+          o << t << "\t" << mean << "\t" << H << "\t" << prev - H << "\t";
+          if (fixed) o << Hf << "\t" << prev - Hf << "\t" << mean_fix;
+          o << std::endl;
+        } else {
+          o << t << "\t" << H << "\t-\n";
+        }
+        break;
+      case EDB_SCHEME_RK2:  // BKW.h:356-381, 579-588, 603-628
+      case EDB_SCHEME_DUFORT:
+        if (k)
+          o << t << "\t" << mean << "\t" << H << "\t" << prev - H << "\t" << std::endl;
+        else
+          o << t << "\t" << mean << "\t" << H << "\t-\n";
+        break;
+      default:  // SSP3, RK4: BKW.h:430-444, 497-514
+        if (k) {
+          o << t << "\t" << H << "\t" << prev - H << "\t";
+          if (fixed) o << Hf << "\t" << prev - Hf;
+          o << std::endl;
+        } else {
+          o << t << "\t" << H << "\t-\n";
+        }
+    }
+  }
+
   void run(int scheme, r_type T0, r_type TEnd, r_type dt, r_type beta) {
+    const bool fixing = (scheme == EDB_SCHEME_RK4 || scheme == EDB_SCHEME_SSP3) ? true : fix_;
     edb_solver_params p{};
     p.N = N0_;
     p.alpha = alpha_;
@@ -153,14 +243,18 @@ class BKW3D {
     p.scheme = scheme;
     p.dt = dt;
     p.beta = beta;
-    p.fix_mode = fix_ ? (applied_ ? EDB_FIX_APPLIED : EDB_FIX_DIAGNOSTIC) : EDB_FIX_NONE;
+    p.fix_mode = fixing ? (applied_ ? EDB_FIX_APPLIED : EDB_FIX_DIAGNOSTIC) : EDB_FIX_NONE;
     p.hermitian = 1;
     p.conserve = conserve_ ? 1 : 0;
     double est = (TEnd - T0) / dt;
     p.history = (int)(est > 0 ? (est < 1e6 ? est : 1e6) : 0) + 16;
+    if (out_) *out_ << "# Initialization...\n" << "# QCalculator calculate B...\n";
     edb_solver* s = nullptr;
-    check(edb_solver_create(&p, nullptr, &s), "BKW3D");
-    long N3 = (long)N0_ * N0_ * N0_;
+    {
+      Watch w{out_};
+      check(edb_solver_create(&p, nullptr, &s), "BKW3D");
+    }
+    const long N3 = (long)N0_ * N0_ * N0_;
     std::vector<r_type> F0(N3);
     r_type h = 2 * R_ / r_type(N0_);
     auto f = init_ ? init_ : std::function<r_type(r_type, r_type, r_type, r_type)>(bkw);
@@ -170,20 +264,103 @@ class BKW3D {
         for (int k = 0; k < N0_; ++k) F0[l++] = f(T0, -R_ + i * h, -R_ + j * h, -R_ + k * h);
     try {
       check(edb_solver_set_initial_real(s, T0, F0.data()), "setInitialValue");
-      int n = 0;
-      check(edb_solver_run(s, TEnd, &n), "run");
-      std::vector<double> H(n), Hf(n);
-      std::vector<int> fx(n);
-      check(edb_solver_entropy_history(s, n < p.history ? n : p.history, H.data(), Hf.data(), fx.data()),
-            "entropy");
-      H_ = Hf;
       F_.resize(N3);
-      check(edb_solver_get_real(s, F_.data()), "F");
+      if (!out_) {
+        int n = 0;
+        check(edb_solver_run(s, TEnd, &n), "run");
+        std::vector<double> H(n), Hf(n);
+        std::vector<int> fx(n);
+        const int kept = n < p.history ? n : p.history;
+        check(edb_solver_entropy_history(s, kept, H.data(), Hf.data(), fx.data()), "entropy");
+        H_.assign(Hf.begin(), Hf.begin() + kept);
+      } else {
+        run_reporting(s, scheme, p, T0, TEnd, dt, F0);
+      }
+      check(edb_solver_get_driver_real(s, F_.data()), "F");
     } catch (...) {
       edb_solver_destroy(s);
       throw;
     }
     edb_solver_destroy(s);
+  }
+
+  // Stepping with the reference's output: one step at a time when FEuler
+  // writes F.dat, else the whole run and the table from the device history.
+  void run_reporting(edb_solver* s, int scheme, const edb_solver_params& p, r_type T0, r_type TEnd, r_type dt,
+                     const std::vector<r_type>& F0) {
+    std::ostream& o = *out_;
+    const long N3 = (long)N0_ * N0_ * N0_;
+    static const char* kName[] = {"Forward Euler", "RK2", "SSP3", "RK4", "C-Forward Euler", "DUFORT"};
+    static const char* kHead[] = {
+        "# Step\tt\tentropy_{n}\tentropy_{n-1}-entropy_{n}\tfixedEntropy\tfixedDiff\n",
+        "# t\tentropy_{n}\tentropy_{n-1}-entropy_{n}\n",
+        "# t\tentropy_{n}\tentropy_{n-1}-entropy_{n}\tNew entropy\tnew entropy_{n-1}-entropy_{n}\n",
+        "# t\tentropy_{n}\tentropy_{n-1}-entropy_{n}\tNew entropy\tnew entropy_{n-1}-entropy_{n}\n",
+        "# t\taverage\tentropy_{n}\tentropy_{n-1}-entropy_{n}\n",
+        "# t\tentropy_{n}\tentropy_{n-1}-entropy_{n}\n"};
+    const bool dufort = scheme == EDB_SCHEME_DUFORT;
+    o << "# Start the evolution using " << kName[dufort ? EDB_SCHEME_RK2 : scheme] << "...\n"
+      << kHead[dufort ? EDB_SCHEME_RK2 : scheme];
+    Watch w{out_};
+    r_type prev = 1.;  // entropy0 = 1 (BKW.h:478 and the like)
+    int step_no = 0;
+    if (scheme == EDB_SCHEME_FEULER) {  // entropy0 = calEntropy() of the initial F, BKW.h:214-216
+      r_type e = 0.;
+      for (long i = 0; i < N3; ++i) e += std::abs(F0[i]) < 1.0e-6 ? 0. : F0[i] * std::log(F0[i]) - F0[i];
+      prev = e * std::pow(2 * R_, 3.) / r_type(N3);
+      o << std::fixed << std::setprecision(8) << "00\t" << T0 << "\t" << prev << "\t-\t-\t-\t\n";
+      step_no = 1;
+    }
+    std::ofstream writeF;
+    const bool perstep = scheme == EDB_SCHEME_FEULER && writeF_;
+    if (perstep) writeF.open("F.dat");
+    // steps and times of the drivers' loops (DUFORT: 8 RK2 steps of dt/8 first)
+    std::vector<r_type> ts;
+    std::vector<int> phase;
+    {
+      r_type t = T0;
+      if (dufort)
+        for (int k = 0; k < 8; ++k) {
+          t += dt / 8;
+          ts.push_back(t);
+          phase.push_back(0);
+        }
+      do {
+        t += dt;
+        ts.push_back(t);
+        phase.push_back(1);
+      } while (t < TEnd);
+    }
+    const int n = (int)ts.size();
+    if (n > p.history) throw std::runtime_error("BKW3D: run longer than the history kept");
+    std::vector<double> H(n), Hf(n), mean(n), mean_fix(n);
+    std::vector<int> fx(n);
+    std::vector<double> Fs(N3);
+    if (perstep) {
+      for (int k = 0; k < n; ++k) {
+        check(edb_solver_step(s, 1), "step");
+        check(edb_solver_get_driver_real(s, Fs.data()), "F");
+        writeF << ts[k] << "\t";
+        for (long i = 0; i < N3; ++i) writeF << Fs[i] << "\t";
+        writeF << "\n";
+      }
+    } else {
+      check(edb_solver_step(s, n), "step");
+    }
+    check(edb_solver_entropy_history(s, n, H.data(), Hf.data(), fx.data()), "entropy");
+    check(edb_solver_mean_history(s, n, mean.data(), mean_fix.data()), "average");
+    H_.assign(Hf.begin(), Hf.end());
+    int k0 = 0;
+    for (int k = 0; k < n; ++k) {
+      if (dufort && k == 8) {
+        o << "# Start the evolution using DUFORT...\n" << kHead[EDB_SCHEME_DUFORT];
+        k0 = 8;
+      }
+      const int sch = dufort ? (phase[k] ? EDB_SCHEME_DUFORT : EDB_SCHEME_RK2) : scheme;
+      row(sch, k - k0, step_no, ts[k], H[k], Hf[k], fx[k] != 0, mean[k], mean_fix[k], prev);
+      prev = Hf[k];
+      ++step_no;
+    }
   }
 
   int N0_;
@@ -194,6 +371,8 @@ class BKW3D {
   int precision_ = EDB_PREC_FP64_PLAIN;
   std::function<r_type(r_type, r_type, r_type, r_type)> init_;
   std::vector<r_type> F_, H_;
+  std::ostream* out_ = nullptr;
+  bool writeF_ = true;
 };
 
 }  // namespace edb

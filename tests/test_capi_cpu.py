@@ -2,6 +2,7 @@
 every entry point include/edb.h declares; argument validation that happens
 before any device call."""
 import ctypes
+import os
 import re
 
 import pytest
@@ -61,3 +62,16 @@ def test_fft_rejects_bad_lengths():
 def test_package_imports_without_gpu():
     import entropydecreaseboltzmann_amd as edb
     assert edb.nmodes(24) == 23 and edb.nmodes(25) == 25
+
+
+def test_cli_usage_without_arguments():
+    """testBKW3D checks its arguments before touching the GPU (the reference's
+    main reads argv unchecked)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "entropydecreaseboltzmann_amd", "bin", "testBKW3D")
+    assert os.path.exists(exe), "build() builds the command-line drivers"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage: testBKW3D N dt TEnd" in r.stderr
+    r = subprocess.run([exe, "2", "0.01", "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
