@@ -308,14 +308,17 @@ def experiment_bkw(edb, N, with_cpu_steps=None):
     runs = [("gpu_fp64_plain", edb.PREC_FP64_PLAIN, edb.FIX_NONE),
             ("gpu_fp64_plain_fix_diagnostic", edb.PREC_FP64_PLAIN, edb.FIX_DIAGNOSTIC),
             ("gpu_fp64_plain_fix_applied", edb.PREC_FP64_PLAIN, edb.FIX_APPLIED),
-            ("gpu_fp32_beta", edb.PREC_FP32_BATCHED, edb.FIX_NONE),
-            ("gpu_fp32_beta_fix_diagnostic", edb.PREC_FP32_BATCHED, edb.FIX_DIAGNOSTIC)]
+            ("gpu_fp32_beta_plain", edb.PREC_FP32_PLAIN, edb.FIX_NONE),
+            ("gpu_fp32_beta_plain_fix_diagnostic", edb.PREC_FP32_PLAIN, edb.FIX_DIAGNOSTIC),
+            ("gpu_fp32_beta_batched", edb.PREC_FP32_BATCHED, edb.FIX_NONE),
+            ("gpu_fp32_beta_batched_fix_diagnostic", edb.PREC_FP32_BATCHED, edb.FIX_DIAGNOSTIC)]
     q = edb.QCalculator(N, 0, L)
     info = q.info()
     q.close()
     for name, prec, fix in runs:
         r = gpu_bkw_run(edb, N, L, F0, dt, n, prec, fix, K_of)
-        beta_bytes = info["bytes_plain64"] if prec == edb.PREC_FP64_PLAIN else info["bytes_pair32"]
+        beta_bytes = {edb.PREC_FP64_PLAIN: info["bytes_plain64"], edb.PREC_FP32_PLAIN: info["bytes_plain64"] // 2,
+                      edb.PREC_FP32_BATCHED: info["bytes_pair32"]}[prec]
         out[name] = {"vs_bkw_end": err_stats(r["F"], ref_end, h3),
                      "max_linf_vs_bkw": r["max_linf_vs_bkw"],
                      "vs_cpu_fp64_end": err_stats(r["F"], Fcpu, h3),
@@ -324,7 +327,9 @@ def experiment_bkw(edb, N, with_cpu_steps=None):
                      "fix_steps": r["fix_steps"], "wall_s": round(r["wall_s"], 2)}
         prog(f"{name}: done")
     plain = out["gpu_fp64_plain"]["vs_bkw_end"]["linf"]
-    out["fp32_over_fp64_error_vs_bkw"] = round(out["gpu_fp32_beta"]["vs_bkw_end"]["linf"] / plain, 4)
+    out["fp32_over_fp64_error_vs_bkw"] = round(out["gpu_fp32_beta_plain"]["vs_bkw_end"]["linf"] / plain, 4)
+    out["fp32_batched_over_fp64_error_vs_bkw"] = round(
+        out["gpu_fp32_beta_batched"]["vs_bkw_end"]["linf"] / plain, 4)
     return out
 
 

@@ -19,12 +19,12 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (FIX_APPLIED, FIX_DIAGNOSTIC, FIX_NONE, PREC_FP32_BATCHED,
+from ._lib import (FIX_APPLIED, FIX_DIAGNOSTIC, FIX_NONE, PREC_FP32_BATCHED, PREC_FP32_PLAIN,
                    PREC_FP64_PLAIN, SCHEMES, EdbError, check, load)
 
 __all__ = ["QCalculator", "fft3d", "EntropyFix", "Solver", "Server", "EdbError", "load",
            "device_count", "nmodes", "SCHEMES", "FIX_NONE", "FIX_DIAGNOSTIC",
-           "FIX_APPLIED", "PREC_FP64_PLAIN", "PREC_FP32_BATCHED"]
+           "FIX_APPLIED", "PREC_FP64_PLAIN", "PREC_FP32_BATCHED", "PREC_FP32_PLAIN"]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -78,6 +78,18 @@ class QCalculator:
               "calQ")
         return q
 
+    def genB_plain32(self):
+        check(load().edb_qcalc_genB_plain32(self._h), "genB_plain32")
+
+    def calQ_plain32(self, fin):
+        """calQ with the FP32 copy of beta (FP64 products and sums)."""
+        N = self.N
+        f = _c128(fin, (N, N, N))
+        q = np.empty_like(f)
+        check(load().edb_qcalc_calQ_plain32(self._h, _ptr(q.view(np.float64)), _ptr(f.view(np.float64))),
+              "calQ_plain32")
+        return q
+
     def calQ_batched(self, fin, hermitian=False):
         N = self.N
         f = _c128(fin)
@@ -121,6 +133,16 @@ class QCalculator:
         check(load().edb_qcalc_autotune(self._h, int(batch), ctypes.byref(bw), ctypes.byref(ps)),
               "autotune")
         return {"batched_waves": bw.value, "plain_slabs": ps.value}
+
+    def tune_batched(self, schedule, count=0):
+        """schedule 0: stream-K rows (count waves); 1: quads (count workgroups)."""
+        check(load().edb_qcalc_tune_batched(self._h, int(schedule), int(count)), "tune_batched")
+
+    def batched_schedule(self):
+        sc, n = ctypes.c_int(), ctypes.c_int()
+        check(load().edb_qcalc_batched_schedule(self._h, ctypes.byref(sc), ctypes.byref(n)),
+              "batched_schedule")
+        return sc.value, n.value
 
     def set_conservative(self, on=True):
         """Moment correction of every Q this calculator returns (zero discrete

@@ -70,13 +70,33 @@ struct BatchedArgs {
   double2* q;
   double2* slots;
   int* wstart;  // [nwaves] item containing each wave's first step
+  float* sink;  // null: keeps the beta prefetch loads alive (never written)
+  long nbeta;   // beta blocks allocated (the prefetch never reads past them)
   int hermitian;
 };
+
+// Touch the next beta block while the current one is computed: one lane per
+// 128-byte L2 line issues a plain vector load, so the next block's scalar
+// loads hit L2 instead of waiting on HBM (beta is streamed once per launch
+// when there is one instance group).  The value is folded into a register
+// that is stored only through a null sink pointer.
+template <int N>
+__device__ __forceinline__ float touch_block(const float* __restrict__ blk, int lane) {
+  constexpr int n = (N - 1) / 2, M = 2 * n + 1, LINES = (M * M * 4 + 127) / 128 + 1;
+  static_assert(LINES <= 64, "block larger than one wave's prefetch");
+  float v = 0.f;
+  if (lane < LINES) {
+    const int off = lane * 32 < M * M ? lane * 32 : M * M - 1;
+    v = blk[off];
+  }
+  return v;
+}
 
 template <int N, int KPT, int H>
 __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__ fa,
                                          const float2* __restrict__ fcr, int Bp, float (&pr)[KPT],
-                                         float (&pi)[KPT]) {
+                                         float (&pi)[KPT], const float* __restrict__ bnext_blk, int lane,
+                                         float& touched) {
   constexpr int n = (N - 1) / 2, M = 2 * n + 1;
   constexpr int K0 = H * KPT, K1 = (H + 1) * KPT < M ? (H + 1) * KPT : M;
   constexpr int NK = K1 - K0;
@@ -85,6 +105,8 @@ __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__
   for (int m = 0; m < M; ++m) c[m] = fcr[(long)m * Bp];
 #pragma unroll
   for (int m = 0; m < M; ++m) a[m] = fa[(long)m * Bp];
+  // issued after the f rows, so their waits do not cover it; consumed at the end
+  const float tv = bnext_blk ? touch_block<N>(bnext_blk, lane) : 0.f;
 #pragma unroll
   for (int j = 0; j < KPT; ++j) {
     pr[j] = 0.f;
@@ -120,6 +142,7 @@ __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__
 #pragma unroll
     for (int j = 0; j < NK; ++j) bcur[j] = bnext[j];
   }
+  touched = tv;
 }
 
 template <int N, int KPT>
@@ -162,6 +185,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   if (lane == 0) A.wstart[w] = it;
   long pos = a0;
   bool first = true;
+  float touched = 0.f;
   while (pos < b0) {
     const long p0 = prefix[it], p1 = prefix[it + 1];
     const long s0 = pos - p0, s1 = (b0 < p1 ? b0 : p1) - p0;
@@ -182,8 +206,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       const float2* fc = fbase + (long)((m1c * M + m2c) * M) * A.Bp;
       const cfloat_p bb = Bc + blk * (long)(M * M);
       float pr[KPT], pi[KPT];
-      if (KS == 1 || h == 0) row_step<N, KPT, 0>(bb, fa, fc, A.Bp, pr, pi);
-      else row_step<N, KPT, (KS > 1 ? 1 : 0)>(bb, fa, fc, A.Bp, pr, pi);
+      float tv = 0.f;
+      const float* nb = blk + 1 < A.nbeta ? A.B + (blk + 1) * (long)(M * M) : nullptr;
+      if (KS == 1 || h == 0) row_step<N, KPT, 0>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
+      else row_step<N, KPT, (KS > 1 ? 1 : 0)>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
+      touched += tv;
 #pragma unroll
       for (int j = 0; j < KPT; ++j) {
         ar[j] += (double)pr[j];
@@ -201,6 +228,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     pos = p0 + s1;
     ++it;
   }
+  if (A.sink) A.sink[(long)w * 64 + lane] = touched;
 }
 
 __device__ inline long range_start(long total, int nwaves, int w) { return total * w / nwaves; }
@@ -250,6 +278,163 @@ __global__ __launch_bounds__(256) void calq_fixup_kernel(BatchedArgs A, int N, i
   }
 }
 
+// ------------------------------------------------------------------ quads
+// Workgroup schedule for f-row reuse.  A workgroup of Q x KS waves takes a
+// quad of Q consecutive k-rows (all k3 parts): wave (j, h) computes k-row j,
+// k3 part h.  All its waves walk the union of the Q rows' l-row schedules in
+// the same (l1, l2) order: at a step the a-row f[l] is the same for every
+// wave and the partner rows f[k_j - l] of neighbouring k-rows are neighbours
+// that slide by one row per step, so the rows a workgroup touches are reused
+// from L2 instead of each wave streaming its own two rows from the fabric.
+// The union table holds, per step, the beta block of each k-row (-1 where
+// the pair is not in that row's schedule; the wave then skips the step).
+// Load balance is stream-K over (quad, instance group, union step) at
+// workgroup granularity, with the partial items at range ends summed by
+// calq_quad_fixup_kernel in block order.
+constexpr int kQuad = 4;
+
+struct QuadArgs {
+  const float* B;
+  const uint32_t* sched;
+  const int4* quadinfo;  // k1c, k2c of the first row (unused), rows in the quad, union offset
+  const int* rowidx;     // [quad][kQuad] rowinfo index of each row (-1 past the end)
+  const int* ublk;       // [union step][kQuad] beta block or -1
+  const long* prefix;
+  int nitems, groups;
+  long total;
+  int nblocks;
+  const float2* fin;
+  int Bp;
+  double2* q;
+  double2* slots;
+  int* bstart;  // [nblocks] item containing each block's first step
+  float* sink;  // null: keeps the beta prefetch loads alive (never written)
+  long nbeta;   // beta blocks allocated (the prefetch never reads past them)
+  const int4* rowinfo;
+  int hermitian;
+};
+
+template <int N, int KS>
+__global__ __launch_bounds__(64 * kQuad * KS) __attribute__((amdgpu_waves_per_eu(2, 2))) void calq_quad_kernel(
+    QuadArgs A) {
+  constexpr int n = (N - 1) / 2, M = 2 * n + 1, KPT = (M + KS - 1) / KS;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int j = wave % kQuad, h = wave / kQuad;
+  const int blk = blockIdx.x;
+  if (blk >= A.nblocks) return;
+  const long a0 = A.total * blk / A.nblocks, b0 = A.total * (blk + 1) / A.nblocks;
+  const clong_p prefix = (clong_p)A.prefix;
+  const cint_p quadinfo = (cint_p)A.quadinfo;
+  const cint_p rowidx = (cint_p)A.rowidx;
+  const cint_p rowinfo = (cint_p)A.rowinfo;
+  const cint_p ublk = (cint_p)A.ublk;
+  const cu32_p sched = (cu32_p)A.sched;
+  const cfloat_p Bc = (cfloat_p)A.B;
+  int lo = 0, hi = A.nitems;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= a0) lo = mid;
+    else hi = mid;
+  }
+  int it = lo;
+  if (threadIdx.x == 0) A.bstart[blk] = it;
+  long pos = a0;
+  bool first = true;
+  float touched = 0.f;
+  while (pos < b0) {
+    const long p0 = prefix[it], p1 = prefix[it + 1];
+    const long s0 = pos - p0, s1 = (b0 < p1 ? b0 : p1) - p0;
+    const int qd = it / A.groups, g = it % A.groups;
+    const int nrows = quadinfo[4 * qd + 2], uoff = quadinfo[4 * qd + 3];
+    double ar[KPT], ai[KPT];
+#pragma unroll
+    for (int t = 0; t < KPT; ++t) {
+      ar[t] = 0.;
+      ai[t] = 0.;
+    }
+    if (j < nrows) {
+      const float2* fbase = A.fin + (long)g * 64 + lane;
+      for (long st = s0; st < s1; ++st) {
+        const int b = ublk[(uoff + st) * kQuad + j];
+        if (b < 0) continue;
+        const uint32_t e = sched[b];
+        const int l1c = e & 255, l2c = (e >> 8) & 255, m1c = (e >> 16) & 255, m2c = e >> 24;
+        const float2* fa = fbase + (long)((l1c * M + l2c) * M) * A.Bp;
+        const float2* fc = fbase + (long)((m1c * M + m2c) * M) * A.Bp;
+        const cfloat_p bb = Bc + (long)b * (M * M);
+        float pr[KPT], pi[KPT];
+        float tv = 0.f;
+        // this row's blocks are consecutive in its schedule: b + 1 is its next one
+        const float* nb = b + 1 < A.nbeta ? A.B + (b + 1) * (long)(M * M) : nullptr;
+        if (KS == 1 || h == 0) row_step<N, KPT, 0>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
+        else row_step<N, KPT, (KS > 1 ? 1 : 0)>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
+        touched += tv;
+#pragma unroll
+        for (int t = 0; t < KPT; ++t) {
+          ar[t] += (double)pr[t];
+          ai[t] += (double)pi[t];
+        }
+      }
+    }
+    if (s0 == 0 && s1 == p1 - p0) {
+      if (j < nrows) {
+        const int r = rowidx[qd * kQuad + j];
+        BatchedArgs W;
+        W.q = A.q;
+        W.Bp = A.Bp;
+        W.hermitian = A.hermitian;
+        write_row<N, KPT>(W, rowinfo[4 * r], rowinfo[4 * r + 1], h, g, lane, ar, ai);
+      }
+    } else {
+      double2* sl = A.slots + ((long)(2 * blk + (first ? 0 : 1)) * (kQuad * KS) + wave) * KPT * 64 + lane;
+#pragma unroll
+      for (int t = 0; t < KPT; ++t) sl[t * 64] = make_double2(ar[t], ai[t]);
+    }
+    first = false;
+    pos = p0 + s1;
+    ++it;
+  }
+  if (A.sink) A.sink[((long)blk * (kQuad * KS) + wave) * 64 + lane] = touched;
+}
+
+// One workgroup (kQuad x KS waves) per interior block boundary; the item the
+// boundary splits is summed, per wave (k-row j, k3 part h), over the slots of
+// the blocks that cover it, by the block of the first such boundary.
+__global__ __launch_bounds__(256 * 2) void calq_quad_fixup_kernel(QuadArgs A, int N, int KS) {
+  const int n = (N - 1) / 2, M = 2 * n + 1, KPT = (M + KS - 1) / KS;
+  const int wb = blockIdx.x + 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = wave % kQuad, h = wave / kQuad;
+  if (wb >= A.nblocks || wave >= kQuad * KS) return;
+  const long p = range_start(A.total, A.nblocks, wb);
+  const int it = A.bstart[wb];
+  const long lo = A.prefix[it], hi = A.prefix[it + 1];
+  if (lo == p || range_start(A.total, A.nblocks, wb - 1) > lo) return;
+  const int bf = wave_of(A.total, A.nblocks, lo), bl = wave_of(A.total, A.nblocks, hi - 1);
+  const int qd = it / A.groups, g = it % A.groups;
+  const int nrows = A.quadinfo[qd].z;
+  if (j >= nrows) return;
+  const int4 ri = A.rowinfo[A.rowidx[qd * kQuad + j]];
+  const bool mirror = A.hermitian && !(ri.x == n && ri.y == n);
+  const long col = (long)g * 64 + lane;
+  for (int t = 0; t < KPT; ++t) {
+    const int k3 = h * KPT + t;
+    if (k3 >= M) break;
+    double sr = 0., si = 0.;
+    for (int b = bf; b <= bl; ++b) {
+      const int slot = range_start(A.total, A.nblocks, b) >= lo ? 0 : 1;
+      const double2 v = A.slots[((long)(2 * b + slot) * (kQuad * KS) + wave) * KPT * 64 + (long)t * 64 + lane];
+      sr += v.x;
+      si += v.y;
+    }
+    A.q[(((long)ri.x * M + ri.y) * M + k3) * A.Bp + col] = make_double2(sr, si);
+    if (mirror)
+      A.q[(((long)(M - 1 - ri.x) * M + (M - 1 - ri.y)) * M + (M - 1 - k3)) * A.Bp + col] =
+          make_double2(sr, -si);
+  }
+}
+
 typedef void (*BatchedFn)(BatchedArgs);
 
 // k3 rows are split in two for the largest grids to keep 2 waves per SIMD.
@@ -274,9 +459,31 @@ BatchedFn lookup(int N, int ks) {
   }
 }
 
+typedef void (*QuadFn)(QuadArgs);
+
+template <int N>
+QuadFn pick_quad(int ks) {
+  return ks == ksplit_of(N) ? calq_quad_kernel<N, ksplit_of(N)> : nullptr;
+}
+
+QuadFn lookup_quad(int N, int ks) {
+  switch (N) {
+#define EDB_CASE(NN) \
+  case NN:           \
+    return pick_quad<NN>(ks);
+    EDB_CASE(3) EDB_CASE(4) EDB_CASE(5) EDB_CASE(6) EDB_CASE(7) EDB_CASE(8) EDB_CASE(9)
+    EDB_CASE(10) EDB_CASE(11) EDB_CASE(12) EDB_CASE(13) EDB_CASE(15) EDB_CASE(16) EDB_CASE(17)
+    EDB_CASE(21) EDB_CASE(23) EDB_CASE(24) EDB_CASE(25) EDB_CASE(31) EDB_CASE(32) EDB_CASE(33)
+#undef EDB_CASE
+    default:
+      return nullptr;
+  }
+}
+
 }  // namespace
 
 int batched_ksplit(int N) { return ksplit_of(N); }
+int quad_rows() { return kQuad; }
 
 // Doubles of the stream-K partial slots of nwaves waves (2 slots of KPT x 64
 // complex each); the per-wave start items follow them as ints.
@@ -284,6 +491,10 @@ size_t batched_slot_doubles(int N, int nwaves) {
   const int M = 2 * ((N - 1) / 2) + 1, ks = ksplit_of(N), kpt = (M + ks - 1) / ks;
   return (size_t)nwaves * 2 * kpt * 64 * 2;
 }
+
+// Slots of the quad schedule: 2 per wave of every block (same count as
+// stream-K over nblocks * kQuad * KS waves).
+int quad_waves_per_block(int N) { return kQuad * ksplit_of(N); }
 bool batched_supported(int N) { return lookup(N, ksplit_of(N)) != nullptr; }
 
 // Host-side schedule: rows ordered Hermitian half first; per row the l-rows
@@ -329,10 +540,10 @@ int batched_prefix(const std::vector<int4>& rowinfo, int nrows, int groups, int 
   return kOk;
 }
 
-int launch_calq_batched_impl(int N, const float* d_B, const uint32_t* d_sched, const int4* d_rowinfo,
-                             const long* d_prefix, int nitems, long total, const float* d_fin, int Bp,
-                             int groups, int hermitian, double* d_q, double* d_slots, int nwaves,
-                             hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+int launch_calq_batched_impl(int N, const float* d_B, long nbeta, const uint32_t* d_sched,
+                             const int4* d_rowinfo, const long* d_prefix, int nitems, long total,
+                             const float* d_fin, int Bp, int groups, int hermitian, double* d_q,
+                             double* d_slots, int nwaves, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   const int ks = batched_ksplit(N);
   BatchedFn fn = lookup(N, ks);
   if (!fn) {
@@ -347,6 +558,8 @@ int launch_calq_batched_impl(int N, const float* d_B, const uint32_t* d_sched, c
   A.nitems = nitems; A.groups = groups; A.total = total; A.nwaves = nwaves;
   A.fin = (const float2*)d_fin; A.Bp = Bp; A.q = (double2*)d_q; A.slots = (double2*)d_slots;
   A.wstart = (int*)(d_slots + batched_slot_doubles(N, nwaves));
+  A.sink = nullptr;
+  A.nbeta = nbeta;
   A.hermitian = hermitian;
   const int wpb = 4;
   if (ev0) EDB_HIP_TRY(hipEventRecord(ev0, s));
@@ -355,6 +568,99 @@ int launch_calq_batched_impl(int N, const float* d_B, const uint32_t* d_sched, c
   if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
   if (nwaves > 1) {
     hipLaunchKernelGGL(calq_fixup_kernel, dim3(nwaves - 1), dim3(256), 0, s, A, N, ks);
+    EDB_HIP_TRY(hipGetLastError());
+  }
+  return kOk;
+}
+
+// Quads: chunks of kQuad consecutive rows of rowinfo's order (Hermitian half
+// first, then the rest; a chunk never mixes the two), and for each quad the
+// union of its rows' l-row schedules in (l1, l2) order with each row's beta
+// block at every union step (-1 where the row has no such pair).
+int build_quad_plan(int N, const std::vector<int4>& rowinfo, const std::vector<uint32_t>& sched,
+                    int nrows_half, std::vector<int4>* quadinfo, std::vector<int>* rowidx,
+                    std::vector<int>* ublk, int* nquads_half) {
+  const int M = 2 * ((N - 1) / 2) + 1;
+  quadinfo->clear();
+  rowidx->clear();
+  ublk->clear();
+  auto add_range = [&](int r0, int r1) {
+    for (int q0 = r0; q0 < r1; q0 += kQuad) {
+      const int nr = std::min(kQuad, r1 - q0);
+      const int uoff = (int)(ublk->size() / kQuad);
+      int cur[kQuad] = {0, 0, 0, 0};
+      for (int l1c = 0; l1c < M; ++l1c)
+        for (int l2c = 0; l2c < M; ++l2c) {
+          int b[kQuad] = {-1, -1, -1, -1};
+          bool any = false;
+          for (int jj = 0; jj < nr; ++jj) {
+            const int4 ri = rowinfo[q0 + jj];
+            if (cur[jj] < ri.w) {
+              const uint32_t e = sched[ri.z + cur[jj]];
+              if ((int)(e & 255) == l1c && (int)((e >> 8) & 255) == l2c) {
+                b[jj] = ri.z + cur[jj];
+                ++cur[jj];
+                any = true;
+              }
+            }
+          }
+          if (any) ublk->insert(ublk->end(), b, b + kQuad);
+        }
+      for (int jj = 0; jj < nr; ++jj)
+        if (cur[jj] != rowinfo[q0 + jj].w) {
+          set_error("quad plan: schedule not in (l1, l2) order");
+          return kInvalid;
+        }
+      quadinfo->push_back(make_int4(rowinfo[q0].x, rowinfo[q0].y, nr, uoff));
+      for (int jj = 0; jj < kQuad; ++jj) rowidx->push_back(jj < nr ? q0 + jj : -1);
+    }
+    return kOk;
+  };
+  EDB_TRY(add_range(0, nrows_half));
+  *nquads_half = (int)quadinfo->size();
+  EDB_TRY(add_range(nrows_half, (int)rowinfo.size()));
+  // sentinel: the union offset one past the last quad
+  quadinfo->push_back(make_int4(0, 0, 0, (int)(ublk->size() / kQuad)));
+  return kOk;
+}
+
+int quad_prefix(const std::vector<int4>& quadinfo, int nquads, int groups, std::vector<long>* prefix) {
+  prefix->assign(1, 0);
+  for (int qd = 0; qd < nquads; ++qd)
+    for (int g = 0; g < groups; ++g)
+      prefix->push_back(prefix->back() + (quadinfo[qd + 1].w - quadinfo[qd].w));
+  return kOk;
+}
+
+int launch_calq_quad_impl(int N, const float* d_B, long nbeta, const uint32_t* d_sched, const int4* d_rowinfo,
+                          const int4* d_quadinfo, const int* d_rowidx, const int* d_ublk, const long* d_prefix,
+                          int nitems, long total, const float* d_fin, int Bp, int groups, int hermitian,
+                          double* d_q, double* d_slots, int nblocks, hipStream_t s, hipEvent_t ev0,
+                          hipEvent_t ev1) {
+  const int ks = batched_ksplit(N);
+  QuadFn fn = lookup_quad(N, ks);
+  if (!fn) {
+    set_error("calq_quad: no compiled kernel for this N");
+    return kUnsupported;
+  }
+  EDB_REQUIRE(Bp % 64 == 0 && groups * 64 <= Bp && groups >= 1, "calq_quad: bad batch padding");
+  EDB_REQUIRE(nblocks >= 1 && total >= 1, "calq_quad: empty launch");
+  if (nblocks > total) nblocks = (int)total;
+  const int wpb = kQuad * ks;
+  QuadArgs A;
+  A.B = d_B; A.sched = d_sched; A.quadinfo = d_quadinfo; A.rowidx = d_rowidx; A.ublk = d_ublk;
+  A.prefix = d_prefix; A.nitems = nitems; A.groups = groups; A.total = total; A.nblocks = nblocks;
+  A.fin = (const float2*)d_fin; A.Bp = Bp; A.q = (double2*)d_q; A.slots = (double2*)d_slots;
+  A.bstart = (int*)(d_slots + batched_slot_doubles(N, nblocks * wpb));
+  A.rowinfo = d_rowinfo; A.hermitian = hermitian;
+  A.sink = nullptr;
+  A.nbeta = nbeta;
+  if (ev0) EDB_HIP_TRY(hipEventRecord(ev0, s));
+  hipLaunchKernelGGL(fn, dim3(nblocks), dim3(64 * wpb), 0, s, A);
+  EDB_HIP_TRY(hipGetLastError());
+  if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
+  if (nblocks > 1) {
+    hipLaunchKernelGGL(calq_quad_fixup_kernel, dim3(nblocks - 1), dim3(64 * wpb), 0, s, A, N, ks);
     EDB_HIP_TRY(hipGetLastError());
   }
   return kOk;

@@ -3,7 +3,8 @@
 //
 // This is the reference algorithm on the GPU with no symmetry reduction: every
 // one of the M^6 FP64 beta entries is streamed once from HBM, so the kernel
-// is HBM-bound (8 bytes and 10 flops per term).
+// is HBM-bound (8 bytes and 10 flops per term).  The same kernel also streams
+// an FP32 copy of beta (4 bytes per term; products and sums stay FP64).
 //
 // Layout.  beta is the reference's row-major [l][k] (rows padded to a multiple
 // of 64): a wave owns 64 consecutive centred k (one per lane) and walks a slab
@@ -36,8 +37,8 @@ __global__ void build_dup_kernel(int N, const double2* __restrict__ fc, int stri
   dup[idx] = v;
 }
 
-template <int UNROLL>
-__global__ __launch_bounds__(64) void calq_plain64_kernel(int M, long NQ, const double* __restrict__ B,
+template <int UNROLL, typename T>
+__global__ __launch_bounds__(64) void calq_plain64_kernel(int M, long NQ, const T* __restrict__ B,
                                                           const double2* __restrict__ dup,
                                                           const double2* __restrict__ fc, int stride,
                                                           double2* __restrict__ part, int nslab) {
@@ -51,7 +52,7 @@ __global__ __launch_bounds__(64) void calq_plain64_kernel(int M, long NQ, const 
   long dbase = ((long)(k1 + M - 1) * W + (k2 + M - 1)) * W + (k3 + M - 1);
   long l0 = NQ * slab / nslab, l1e = NQ * (slab + 1) / nslab;
   const long kp = (long)gridDim.x * 64;  // padded row length
-  const double* bt = B + (long)tile * 64 + lane;
+  const T* bt = B + (long)tile * 64 + lane;
   double qr = 0., qi = 0.;
   int a1 = (int)(l0 / ((long)M * M)), a2 = (int)((l0 / M) % M), a3 = (int)(l0 % M);
   long l = l0;
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(64) void calq_plain64_kernel(int M, long NQ, const 
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
       long lin = ((long)a1 * W + a2) * W + a3;
-      b[u] = __builtin_nontemporal_load(bt + (l + u) * kp);
+      b[u] = (double)__builtin_nontemporal_load(bt + (l + u) * kp);
       c[u] = dup[dbase - lin];
       a[u] = fc[(l + u) * stride];
       if (++a3 == M) { a3 = 0; if (++a2 == M) { a2 = 0; ++a1; } }
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(64) void calq_plain64_kernel(int M, long NQ, const 
   }
   for (; l < l1e; ++l) {
     long lin = ((long)a1 * W + a2) * W + a3;
-    double b = __builtin_nontemporal_load(bt + l * kp);
+    double b = (double)__builtin_nontemporal_load(bt + l * kp);
     double2 c = dup[dbase - lin];
     double2 a = fc[l * stride];
     double pr = a.x * c.x - a.y * c.y;
@@ -117,7 +118,20 @@ int launch_calq_plain64(int N, const double* d_B, const double* d_dup, const dou
   Geometry g(N);
   EDB_REQUIRE(nslab >= 1 && nslab <= 4096 && nslab <= g.NQ, "calq_plain64: bad slab count");
   int ntile = (int)((g.NQ + 63) / 64);
-  hipLaunchKernelGGL(calq_plain64_kernel<8>, dim3(ntile, nslab), dim3(64), 0, s, g.M, g.NQ, d_B,
+  hipLaunchKernelGGL((calq_plain64_kernel<8, double>), dim3(ntile, nslab), dim3(64), 0, s, g.M, g.NQ, d_B,
+                     (const double2*)d_dup, (const double2*)d_fc, fc_stride, (double2*)d_part, nslab);
+  EDB_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+// Same evaluation with the FP32 copy of beta (products and sums stay FP64):
+// half the bytes of the HBM stream.
+int launch_calq_plain32(int N, const float* d_B, const double* d_dup, const double* d_fc, int fc_stride,
+                        double* d_part, int nslab, hipStream_t s) {
+  Geometry g(N);
+  EDB_REQUIRE(nslab >= 1 && nslab <= 4096 && nslab <= g.NQ, "calq_plain32: bad slab count");
+  int ntile = (int)((g.NQ + 63) / 64);
+  hipLaunchKernelGGL((calq_plain64_kernel<8, float>), dim3(ntile, nslab), dim3(64), 0, s, g.M, g.NQ, d_B,
                      (const double2*)d_dup, (const double2*)d_fc, fc_stride, (double2*)d_part, nslab);
   EDB_HIP_TRY(hipGetLastError());
   return kOk;

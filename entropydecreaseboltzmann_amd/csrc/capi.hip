@@ -45,12 +45,17 @@ int ensure_tables(edb_qcalc* q) {
 }
 
 int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out) {
-  auto key = std::make_pair(groups, hermitian);
+  auto key = std::make_pair(groups, hermitian + 2 * q->schedule);
   auto it = q->prefixes.find(key);
   if (it == q->prefixes.end()) {
-    int nrows = hermitian ? q->nrows_half : (int)q->rowinfo.size();
     std::vector<long> pre;
-    batched_prefix(q->rowinfo, nrows, groups, batched_ksplit(q->N), &pre);
+    if (q->schedule == 1) {
+      int nquads = hermitian ? q->nquads_half : (int)q->quadinfo.size() - 1;
+      quad_prefix(q->quadinfo, nquads, groups, &pre);
+    } else {
+      int nrows = hermitian ? q->nrows_half : (int)q->rowinfo.size();
+      batched_prefix(q->rowinfo, nrows, groups, batched_ksplit(q->N), &pre);
+    }
     PrefixEntry e;
     e.nitems = (int)pre.size() - 1;
     e.total = pre.back();
@@ -83,6 +88,23 @@ int ensure_moment_table(edb_qcalc* q) {
   return kOk;
 }
 
+int batched_shape(edb_qcalc* q, int groups, int hermitian, PrefixEntry** pe, int* count) {
+  EDB_TRY(get_prefix(q, groups, hermitian ? 1 : 0, pe));
+  if (q->schedule == 1) {
+    const int wpb = quad_waves_per_block(q->N);
+    int nb = q->nblocks_quad > 0 ? q->nblocks_quad : cu_count() * 16 / wpb;
+    if (nb > (*pe)->total) nb = (int)(*pe)->total;
+    EDB_TRY(ensure_slots(q, nb * wpb));
+    *count = nb;
+  } else {
+    int nwaves = q->nwaves > 0 ? q->nwaves : cu_count() * 8;
+    if (nwaves > (*pe)->total) nwaves = (int)(*pe)->total;
+    EDB_TRY(ensure_slots(q, nwaves));
+    *count = nwaves;
+  }
+  return kOk;
+}
+
 // Batched Q on device arrays already in the kernel layout.
 int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, const float* d_fin,
                    hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
@@ -90,13 +112,15 @@ int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, 
   EDB_REQUIRE(batch >= 1 && ld % 64 == 0 && batch <= ld, "calQ_batched: bad batch / ld");
   const int groups = (batch + 63) / 64;
   PrefixEntry* pe = nullptr;
-  EDB_TRY(get_prefix(q, groups, hermitian ? 1 : 0, &pe));
-  int nwaves = q->nwaves > 0 ? q->nwaves : cu_count() * 8;
-  if (nwaves > pe->total) nwaves = (int)pe->total;
-  EDB_TRY(ensure_slots(q, nwaves));
-  return launch_calq_batched_impl(q->N, q->d_pair, q->d_sched, q->d_rowinfo, pe->d_prefix, pe->nitems,
+  int count = 0;
+  EDB_TRY(batched_shape(q, groups, hermitian, &pe, &count));
+  if (q->schedule == 1)
+    return launch_calq_quad_impl(q->N, q->d_pair, q->nblocks, q->d_sched, q->d_rowinfo, q->d_quadinfo, q->d_rowidx,
+                                 q->d_ublk, pe->d_prefix, pe->nitems, pe->total, d_fin, ld, groups,
+                                 hermitian ? 1 : 0, d_q, q->d_slots, count, s, ev0, ev1);
+  return launch_calq_batched_impl(q->N, q->d_pair, q->nblocks, q->d_sched, q->d_rowinfo, pe->d_prefix, pe->nitems,
                                   pe->total, d_fin, ld, groups, hermitian ? 1 : 0, d_q, q->d_slots,
-                                  nwaves, s, ev0, ev1);
+                                  count, s, ev0, ev1);
 }
 
 int ensure_plain_scratch(edb_qcalc* q) {
@@ -110,12 +134,16 @@ int ensure_plain_scratch(edb_qcalc* q) {
 // Plain FP64 Q of one column: d_fc FP64 centred [M^3] with stride fc_stride
 // (complex elements), result centred into d_q with stride q_stride.
 int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, int q_stride,
-                 hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
-  EDB_REQUIRE(q->d_plain, "calQ: genB has not run");
+                 hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, bool fp32) {
+  if (fp32) EDB_REQUIRE(q->d_plain32, "calQ_plain32: genB_plain32 has not run");
+  else EDB_REQUIRE(q->d_plain, "calQ: genB has not run");
   EDB_TRY(ensure_plain_scratch(q));
   EDB_TRY(launch_build_dup(q->N, d_fc, fc_stride, q->d_dup, s));
   if (ev0) EDB_HIP_TRY(hipEventRecord(ev0, s));
-  EDB_TRY(launch_calq_plain64(q->N, q->d_plain, q->d_dup, d_fc, fc_stride, q->d_part, q->nslab, s));
+  if (fp32)
+    EDB_TRY(launch_calq_plain32(q->N, q->d_plain32, q->d_dup, d_fc, fc_stride, q->d_part, q->nslab, s));
+  else
+    EDB_TRY(launch_calq_plain64(q->N, q->d_plain, q->d_dup, d_fc, fc_stride, q->d_part, q->nslab, s));
   if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
   EDB_TRY(launch_reduce_plain(q->N, q->d_part, q->nslab, d_q, q_stride, s));
   return kOk;
@@ -163,6 +191,9 @@ int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out) {
   // enough l-slabs to give every SIMD several waves of the HBM stream
   q->nslab = std::max(1, std::min(64, (cu_count() * 12 + q->ntile - 1) / q->ntile));
   q->nslab = (int)std::min<long>(q->nslab, q->g.NQ);
+  // measured default: workgroup quads for the k3-split grids (N >= 27), where
+  // they beat stream-K rows by ~10%; rows elsewhere (the autotuner times both)
+  q->schedule = batched_ksplit(N) > 1 ? EDB_SCHED_QUADS : EDB_SCHED_ROWS;
   hipError_t e = hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -175,7 +206,7 @@ int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out) {
 
 int edb_qcalc_close_filter(edb_qcalc* q) {
   EDB_REQUIRE(q, "null handle");
-  EDB_REQUIRE(!q->d_plain && !q->d_pair, "closeFilter must precede genB");
+  EDB_REQUIRE(!q->d_plain && !q->d_plain32 && !q->d_pair, "closeFilter must precede genB");
   q->filter = 0;
   if (q->tables_ready) {
     beta_tables_free(&q->tables);
@@ -191,6 +222,17 @@ int edb_qcalc_genB(edb_qcalc* q) {
   EDB_TRY(ensure_tables(q));
   EDB_TRY(dmalloc(&q->d_plain, (size_t)q->ntile * q->g.NQ * 64));
   EDB_TRY(launch_genb_plain64(q->tables, q->d_plain, q->ntile, q->stream));
+  EDB_HIP_TRY(hipStreamSynchronize(q->stream));
+  return EDB_OK;
+}
+
+int edb_qcalc_genB_plain32(edb_qcalc* q) {
+  EDB_REQUIRE(q, "null handle");
+  std::lock_guard<std::mutex> lk(q->mu);
+  if (q->d_plain32) return EDB_OK;
+  EDB_TRY(ensure_tables(q));
+  EDB_TRY(dmalloc(&q->d_plain32, (size_t)q->ntile * q->g.NQ * 64));
+  EDB_TRY(launch_genb_plain32(q->tables, q->d_plain32, q->ntile, q->stream));
   EDB_HIP_TRY(hipStreamSynchronize(q->stream));
   return EDB_OK;
 }
@@ -212,6 +254,16 @@ int edb_qcalc_genB_batched(edb_qcalc* q) {
   EDB_HIP_TRY(hipMemcpy(q->d_rowinfo, q->rowinfo.data(), sizeof(int4) * q->rowinfo.size(),
                         hipMemcpyHostToDevice));
   EDB_HIP_TRY(hipMemcpy(q->d_sched, sched.data(), sizeof(uint32_t) * sched.size(), hipMemcpyHostToDevice));
+  std::vector<int> rowidx, ublk;
+  EDB_TRY(build_quad_plan(q->N, q->rowinfo, sched, q->nrows_half, &q->quadinfo, &rowidx, &ublk,
+                          &q->nquads_half));
+  EDB_TRY(dmalloc(&q->d_quadinfo, q->quadinfo.size()));
+  EDB_TRY(dmalloc(&q->d_rowidx, rowidx.size()));
+  EDB_TRY(dmalloc(&q->d_ublk, ublk.size()));
+  EDB_HIP_TRY(hipMemcpy(q->d_quadinfo, q->quadinfo.data(), sizeof(int4) * q->quadinfo.size(),
+                        hipMemcpyHostToDevice));
+  EDB_HIP_TRY(hipMemcpy(q->d_rowidx, rowidx.data(), sizeof(int) * rowidx.size(), hipMemcpyHostToDevice));
+  EDB_HIP_TRY(hipMemcpy(q->d_ublk, ublk.data(), sizeof(int) * ublk.size(), hipMemcpyHostToDevice));
   EDB_TRY(dmalloc(&q->d_pair, (size_t)q->nblocks * q->g.M * q->g.M));
   EDB_TRY(launch_genb_pair32(q->tables, (const int*)q->d_rowinfo, q->d_sched, (int)q->rowinfo.size(),
                              q->nblocks, q->d_pair, q->stream));
@@ -219,14 +271,15 @@ int edb_qcalc_genB_batched(edb_qcalc* q) {
   return EDB_OK;
 }
 
-int edb_qcalc_calQ_device(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream) {
+namespace {
+int calq_device_impl(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream, bool fp32) {
   EDB_REQUIRE(q && d_qout && d_fin, "calQ_device: null argument");
   hipStream_t s = stream ? (hipStream_t)stream : q->stream;
   double *fc = nullptr, *qc = nullptr;
   EDB_TRY(dmalloc(&fc, (size_t)q->g.NQ * 2));
   int st = dmalloc(&qc, (size_t)q->g.NQ * 2);
   if (st == kOk) st = launch_pack(q->N, 1, 1, false, fc, d_fin, nullptr, 0., s);
-  if (st == kOk) st = plain_column(q, fc, 1, qc, 1, s);
+  if (st == kOk) st = plain_column(q, fc, 1, qc, 1, s, nullptr, nullptr, fp32);
   if (st == kOk && q->conserve) st = launch_moment_correct(q->d_mom, q->mom_E, 1, 1, qc, s);
   if (st == kOk) st = launch_unpack(q->N, 1, 1, d_qout, qc, s);
   if (st == kOk && hipStreamSynchronize(s) != hipSuccess) {
@@ -238,19 +291,30 @@ int edb_qcalc_calQ_device(edb_qcalc* q, double* d_qout, const double* d_fin, voi
   return st;
 }
 
-int edb_qcalc_calQ(edb_qcalc* q, double* qout, const double* fin) {
+int calq_host_impl(edb_qcalc* q, double* qout, const double* fin, bool fp32) {
   EDB_REQUIRE(q && qout && fin, "calQ: null argument");
   size_t bytes = sizeof(double) * 2 * q->g.N3;
   double *din = nullptr, *dout = nullptr;
   EDB_TRY(dmalloc(&din, (size_t)q->g.N3 * 2));
   int st = dmalloc(&dout, (size_t)q->g.N3 * 2);
   if (st == kOk && hipMemcpy(din, fin, bytes, hipMemcpyHostToDevice) != hipSuccess) st = kHipError;
-  if (st == kOk) st = edb_qcalc_calQ_device(q, dout, din, q->stream);
+  if (st == kOk) st = calq_device_impl(q, dout, din, q->stream, fp32);
   if (st == kOk && hipMemcpy(qout, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = kHipError;
   if (st == kHipError && g_err.empty()) set_error("calQ: copy failed");
   dfree(&din);
   dfree(&dout);
   return st;
+}
+}  // namespace
+
+int edb_qcalc_calQ_device(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream) {
+  return calq_device_impl(q, d_qout, d_fin, stream, false);
+}
+
+int edb_qcalc_calQ(edb_qcalc* q, double* qout, const double* fin) { return calq_host_impl(q, qout, fin, false); }
+
+int edb_qcalc_calQ_plain32(edb_qcalc* q, double* qout, const double* fin) {
+  return calq_host_impl(q, qout, fin, true);
 }
 
 int edb_qcalc_calQ_batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_qout,
@@ -419,22 +483,56 @@ int edb_qcalc_tune(edb_qcalc* q, int batched_waves, int plain_slabs) {
   return EDB_OK;
 }
 
+int edb_qcalc_tune_batched(edb_qcalc* q, int schedule, int count) {
+  EDB_REQUIRE(q, "null handle");
+  EDB_REQUIRE(schedule == EDB_SCHED_ROWS || schedule == EDB_SCHED_QUADS, "tune_batched: unknown schedule");
+  EDB_REQUIRE(count >= 0 && count <= (1 << 20), "tune_batched: count out of range");
+  std::lock_guard<std::mutex> lk(q->mu);
+  q->schedule = schedule;
+  if (schedule == EDB_SCHED_QUADS) q->nblocks_quad = count;
+  else q->nwaves = count;
+  return EDB_OK;
+}
+
+int edb_qcalc_batched_schedule(const edb_qcalc* q, int* schedule, int* count) {
+  EDB_REQUIRE(q, "null handle");
+  if (schedule) *schedule = q->schedule;
+  if (count) *count = q->schedule == EDB_SCHED_QUADS ? q->nblocks_quad : q->nwaves;
+  return EDB_OK;
+}
+
 int edb_qcalc_autotune(edb_qcalc* q, int batch, int* batched_waves, int* plain_slabs) {
   EDB_REQUIRE(q, "null handle");
   const int cu = cu_count();
   if (q->d_pair && batch >= 1) {
     double best = 1e30;
-    int bw = 0;
-    for (int per_cu : {4, 6, 8, 12, 16}) {
+    int bsched = 0, bcount = 0;
+    for (int per_cu : {4, 6, 8, 12, 16}) {  // stream-K rows: persistent waves per CU
+      q->schedule = EDB_SCHED_ROWS;
       q->nwaves = cu * per_cu;
       double ms = 0.;
       EDB_TRY(edb_qcalc_time_batched(q, batch, 1, 3, &ms));
       if (ms < best) {
         best = ms;
-        bw = q->nwaves;
+        bsched = EDB_SCHED_ROWS;
+        bcount = q->nwaves;
       }
     }
-    q->nwaves = bw;
+    const int wpb = quad_waves_per_block(q->N);
+    for (int per_cu : {8, 16, 24, 32}) {  // quads: persistent workgroups, as waves per CU
+      q->schedule = EDB_SCHED_QUADS;
+      q->nblocks_quad = std::max(1, cu * per_cu / wpb);
+      double ms = 0.;
+      EDB_TRY(edb_qcalc_time_batched(q, batch, 1, 3, &ms));
+      if (ms < best) {
+        best = ms;
+        bsched = EDB_SCHED_QUADS;
+        bcount = q->nblocks_quad;
+      }
+    }
+    q->schedule = bsched;
+    if (bsched == EDB_SCHED_QUADS) q->nblocks_quad = bcount;
+    else q->nwaves = bcount;
   }
   if (q->d_plain) {
     double best = 1e30;
@@ -451,7 +549,8 @@ int edb_qcalc_autotune(edb_qcalc* q, int batch, int* batched_waves, int* plain_s
     }
     EDB_TRY(edb_qcalc_tune(q, q->nwaves, bs));
   }
-  if (batched_waves) *batched_waves = q->nwaves;
+  if (batched_waves)
+    *batched_waves = q->schedule == EDB_SCHED_QUADS ? q->nblocks_quad * quad_waves_per_block(q->N) : q->nwaves;
   if (plain_slabs) *plain_slabs = q->nslab;
   return EDB_OK;
 }
@@ -468,12 +567,16 @@ int edb_qcalc_destroy(edb_qcalc* q) {
   if (!q) return EDB_OK;
   if (q->stream) (void)hipStreamSynchronize(q->stream);
   dfree(&q->d_mom);
+  dfree(&q->d_plain32);
   dfree(&q->d_plain);
   dfree(&q->d_dup);
   dfree(&q->d_part);
   dfree(&q->d_pair);
   dfree(&q->d_rowinfo);
   dfree(&q->d_sched);
+  dfree(&q->d_quadinfo);
+  dfree(&q->d_rowidx);
+  dfree(&q->d_ublk);
   dfree(&q->d_slots);
   for (auto& kv : q->prefixes) dfree(&kv.second.d_prefix);
   if (q->tables_ready) beta_tables_free(&q->tables);
@@ -619,6 +722,7 @@ struct edb_solver {
 namespace {
 
 bool batched(const edb_solver* s) { return s->p.precision == EDB_PREC_FP32_BATCHED; }
+bool plain32(const edb_solver* s) { return s->p.precision == EDB_PREC_FP32_PLAIN; }
 
 int next_events(edb_solver* s, hipEvent_t* e0, hipEvent_t* e1) {
   *e0 = *e1 = nullptr;
@@ -647,7 +751,8 @@ int qeval(edb_solver* s, const double* X, const double* Qprev, double c0, double
     for (int i = 0; i < batch; ++i) {
       hipEvent_t e0, e1;
       EDB_TRY(next_events(s, &e0, &e1));
-      EDB_TRY(plain_column(s->qc, (const double*)s->in + 2 * i, ld, Qout + 2 * i, ld, s->stream, e0, e1));
+      EDB_TRY(plain_column(s->qc, (const double*)s->in + 2 * i, ld, Qout + 2 * i, ld, s->stream, e0, e1,
+                           plain32(s)));
     }
   }
   if (s->p.conserve) EDB_TRY(launch_moment_correct(s->qc->d_mom, s->qc->mom_E, batch, ld, Qout, s->stream));
@@ -807,10 +912,8 @@ int prepare_buffers(edb_solver* s) {
   if (batched(s)) {
     const int groups = (s->p.batch + 63) / 64;
     PrefixEntry* pe = nullptr;
-    EDB_TRY(get_prefix(s->qc, groups, s->p.hermitian ? 1 : 0, &pe));
-    int nwaves = s->qc->nwaves > 0 ? s->qc->nwaves : cu_count() * 8;
-    if (nwaves > pe->total) nwaves = (int)pe->total;
-    EDB_TRY(ensure_slots(s->qc, nwaves));
+    int count = 0;
+    EDB_TRY(batched_shape(s->qc, groups, s->p.hermitian ? 1 : 0, &pe, &count));
   } else {
     EDB_TRY(ensure_plain_scratch(s->qc));
   }
@@ -863,7 +966,8 @@ int edb_solver_create(const edb_solver_params* p, edb_qcalc* shared, edb_solver*
   EDB_REQUIRE(p && out, "solver_create: null argument");
   *out = nullptr;
   EDB_REQUIRE(p->batch >= 1 && p->batch <= 65536, "solver_create: batch must be in [1, 65536]");
-  EDB_REQUIRE(p->precision == EDB_PREC_FP64_PLAIN || p->precision == EDB_PREC_FP32_BATCHED,
+  EDB_REQUIRE(p->precision == EDB_PREC_FP64_PLAIN || p->precision == EDB_PREC_FP32_BATCHED ||
+                  p->precision == EDB_PREC_FP32_PLAIN,
               "solver_create: unknown precision");
   EDB_REQUIRE(p->scheme >= 0 && p->scheme <= 5, "solver_create: unknown scheme");
   EDB_REQUIRE(p->fix_mode >= 0 && p->fix_mode <= 2, "solver_create: unknown fix mode");
@@ -888,7 +992,9 @@ int edb_solver_create(const edb_solver_params* p, edb_qcalc* shared, edb_solver*
     s->own_qc = st == kOk;
     if (st == kOk && !p->filter) st = edb_qcalc_close_filter(s->qc);
   }
-  if (st == kOk) st = batched(s) ? edb_qcalc_genB_batched(s->qc) : edb_qcalc_genB(s->qc);
+  if (st == kOk)
+    st = batched(s) ? edb_qcalc_genB_batched(s->qc)
+                    : (plain32(s) ? edb_qcalc_genB_plain32(s->qc) : edb_qcalc_genB(s->qc));
   if (st == kOk && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("solver_create: stream creation failed");
     st = kHipError;

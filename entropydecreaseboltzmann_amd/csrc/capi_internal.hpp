@@ -51,8 +51,9 @@ struct edb_qcalc {
   hipStream_t stream = nullptr;
   edb::BetaTables tables;
   bool tables_ready = false;
-  // plain FP64
+  // plain FP64 (and its FP32 copy)
   double* d_plain = nullptr;
+  float* d_plain32 = nullptr;
   int ntile = 0;
   int nslab = 16;
   double* d_dup = nullptr;
@@ -64,10 +65,18 @@ struct edb_qcalc {
   std::vector<int4> rowinfo;
   int nrows_half = 0;
   long nblocks = 0;
-  std::map<std::pair<int, int>, edb::PrefixEntry> prefixes;
+  std::map<std::pair<int, int>, edb::PrefixEntry> prefixes;  // (groups, hermitian + 2 * schedule)
   double* d_slots = nullptr;
   size_t slots_count = 0;
   int nwaves = 0;
+  // quad schedule (schedule 1): workgroups of 4 neighbouring k-rows
+  int schedule = 0;
+  int nblocks_quad = 0;
+  std::vector<int4> quadinfo;
+  int nquads_half = 0;
+  int4* d_quadinfo = nullptr;
+  int* d_rowidx = nullptr;
+  int* d_ublk = nullptr;
   // conservative moment correction
   int conserve = 0;
   double* d_mom = nullptr;
@@ -80,6 +89,10 @@ namespace edb {
 int ensure_tables(edb_qcalc* q);
 int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out);
 int ensure_slots(edb_qcalc* q, int nwaves);
+// Launch shape of the batched kernel for `groups` instance groups under the
+// handle's schedule: its prefix and persistent waves (rows) or workgroups
+// (quads), with the partial-sum slots allocated.
+int batched_shape(edb_qcalc* q, int groups, int hermitian, PrefixEntry** pe, int* count);
 // Device table of the moment correction for q's (N, L), built on first use.
 int ensure_moment_table(edb_qcalc* q);
 // Batched Q on device arrays already in the kernel layout (float2 [M^3][ld] in,
@@ -88,7 +101,8 @@ int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, 
                    hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 int ensure_plain_scratch(edb_qcalc* q);
 // Plain FP64 Q of one column (centred FP64 input with stride fc_stride).
+// Plain Q of one column with the FP64 beta, or its FP32 copy when fp32.
 int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, int q_stride,
-                 hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                 hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, bool fp32 = false);
 
 }  // namespace edb

@@ -96,11 +96,14 @@ int beta_tables_init(BetaTables* t, int N, int alpha, double L, int filter);
 void beta_tables_free(BetaTables* t);
 int launch_genb_ref64(const BetaTables& t, double* d_B, hipStream_t s);
 int launch_genb_plain64(const BetaTables& t, double* d_B, int ntile, hipStream_t s);
+int launch_genb_plain32(const BetaTables& t, float* d_B, int ntile, hipStream_t s);
 int launch_genb_pair32(const BetaTables& t, const int* d_rowinfo, const uint32_t* d_sched,
                        int nrows, long nblocks, float* d_B, hipStream_t s);
 
 // plain FP64 Q (calq_plain.hip)
 int launch_build_dup(int N, const double* d_fc, int fc_stride, double* d_dup, hipStream_t s);
+int launch_calq_plain32(int N, const float* d_B, const double* d_dup, const double* d_fc, int fc_stride,
+                        double* d_part, int nslab, hipStream_t s);
 int launch_calq_plain64(int N, const double* d_B, const double* d_dup, const double* d_fc,
                         int fc_stride, double* d_part, int nslab, hipStream_t s);
 int launch_reduce_plain(int N, const double* d_part, int nslab, double* d_q, int q_stride,

@@ -98,7 +98,8 @@ __global__ void genb_ref64_kernel(BetaArgs a, double* __restrict__ B, long NQ) {
 
 // Plain-kernel layout: the reference's row-major [l][k] with each row padded
 // to ntile*64 entries (zeros past M^3), k = tile*64 + lane.
-__global__ void genb_plain64_kernel(BetaArgs a, double* __restrict__ B, long NQ, int ntile) {
+template <typename T>
+__global__ void genb_plain_kernel(BetaArgs a, T* __restrict__ B, long NQ, int ntile) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long kp = (long)ntile * 64;
   long total = kp * NQ;
@@ -110,7 +111,7 @@ __global__ void genb_plain64_kernel(BetaArgs a, double* __restrict__ B, long NQ,
   if (kc < NQ)
     v = beta_lk(a, (int)(lc / (M * M)), (int)((lc / M) % M), (int)(lc % M), (int)(kc / (M * M)),
                 (int)((kc / M) % M), (int)(kc % M));
-  B[idx] = v;
+  B[idx] = (T)v;  // FP32 layout: the FP64 value rounded once
 }
 
 // Paired layout: block b holds an M x M table [l3][k3] for the scheduled
@@ -202,7 +203,16 @@ int launch_genb_ref64(const BetaTables& t, double* d_B, hipStream_t s) {
 int launch_genb_plain64(const BetaTables& t, double* d_B, int ntile, hipStream_t s) {
   Geometry g(t.N);
   long total = (long)ntile * g.NQ * 64;
-  hipLaunchKernelGGL(genb_plain64_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s,
+  hipLaunchKernelGGL(genb_plain_kernel<double>, dim3(blocks_for(total, 256)), dim3(256), 0, s,
+                     make_args(t), d_B, g.NQ, ntile);
+  EDB_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+int launch_genb_plain32(const BetaTables& t, float* d_B, int ntile, hipStream_t s) {
+  Geometry g(t.N);
+  long total = (long)ntile * g.NQ * 64;
+  hipLaunchKernelGGL(genb_plain_kernel<float>, dim3(blocks_for(total, 256)), dim3(256), 0, s,
                      make_args(t), d_B, g.NQ, ntile);
   EDB_HIP_TRY(hipGetLastError());
   return kOk;

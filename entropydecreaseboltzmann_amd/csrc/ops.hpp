@@ -97,9 +97,21 @@ bool batched_supported(int N);
 int build_pair_plan(int N, std::vector<int4>* rowinfo, std::vector<uint32_t>* sched, int* nrows_half);
 int batched_prefix(const std::vector<int4>& rowinfo, int nrows, int groups, int ks,
                    std::vector<long>* prefix);
-int launch_calq_batched_impl(int N, const float* d_B, const uint32_t* d_sched, const int4* d_rowinfo,
-                             const long* d_prefix, int nitems, long total, const float* d_fin, int Bp,
-                             int groups, int hermitian, double* d_q, double* d_slots, int nwaves,
-                             hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+int quad_rows();
+int quad_waves_per_block(int N);
+int build_quad_plan(int N, const std::vector<int4>& rowinfo, const std::vector<uint32_t>& sched,
+                    int nrows_half, std::vector<int4>* quadinfo, std::vector<int>* rowidx,
+                    std::vector<int>* ublk, int* nquads_half);
+int quad_prefix(const std::vector<int4>& quadinfo, int nquads, int groups, std::vector<long>* prefix);
+int launch_calq_quad_impl(int N, const float* d_B, long nbeta, const uint32_t* d_sched, const int4* d_rowinfo,
+                          const int4* d_quadinfo, const int* d_rowidx, const int* d_ublk, const long* d_prefix,
+                          int nitems, long total, const float* d_fin, int Bp, int groups, int hermitian,
+                          double* d_q, double* d_slots, int nblocks, hipStream_t s, hipEvent_t ev0 = nullptr,
+                          hipEvent_t ev1 = nullptr);
+int launch_calq_batched_impl(int N, const float* d_B, long nbeta, const uint32_t* d_sched,
+                             const int4* d_rowinfo, const long* d_prefix, int nitems, long total,
+                             const float* d_fin, int Bp, int groups, int hermitian, double* d_q,
+                             double* d_slots, int nwaves, hipStream_t s, hipEvent_t ev0 = nullptr,
+                             hipEvent_t ev1 = nullptr);
 
 }  // namespace edb

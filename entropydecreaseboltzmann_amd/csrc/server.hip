@@ -149,8 +149,8 @@ int make_group(edb_server* s, int N, int alpha, Group** out) {
   if (st == kOk && s->p.conserve) st = ensure_moment_table(g->qc);
   if (st == kOk) {
     PrefixEntry* pe = nullptr;
-    for (int grp = 1; grp <= g->cap / 64 && st == kOk; ++grp) st = get_prefix(g->qc, grp, 1, &pe);
-    if (st == kOk) st = ensure_slots(g->qc, cu_count() * 8);
+    int count = 0;
+    for (int grp = 1; grp <= g->cap / 64 && st == kOk; ++grp) st = batched_shape(g->qc, grp, 1, &pe, &count);
   }
   if (st == kOk) st = launch_fill((double*)g->fixed, g->cap / 2, 0., g->stream);
   if (st == kOk && hipStreamSynchronize(g->stream) != hipSuccess) st = kHipError;

@@ -46,6 +46,7 @@ extern "C" {
 
 #define EDB_PREC_FP64_PLAIN 0    /* one instance at a time, FP64 beta streamed */
 #define EDB_PREC_FP32_BATCHED 1  /* all instances in one launch, FP32 beta */
+#define EDB_PREC_FP32_PLAIN 2    /* one instance at a time, FP32 beta streamed, FP64 products */
 
 #define EDB_SCHEME_FEULER 0  /* BKW3D::FEuler, BKW.h:191-251 */
 #define EDB_SCHEME_RK2 1     /* BKW3D::RK2,    BKW.h:314-384 */
@@ -87,10 +88,16 @@ int edb_qcalc_genB(edb_qcalc* q);
 /* The FP32 paired layout used by the batched kernel (M^6/2 entries); no
  * reference counterpart (the reference has only the FP64 N^6 array). */
 int edb_qcalc_genB_batched(edb_qcalc* q);
+/* FP32 copy of genB's [l][k] beta (the FP64 values rounded once) for the
+ * single-instance FP32-beta path: half the bytes of the plain HBM stream,
+ * products and sums in FP64.  No reference counterpart (its B is FP64). */
+int edb_qcalc_genB_plain32(edb_qcalc* q);
 
 /* calQ(Qout, fin), QCalculator.h:59, 231-257: one Q-evaluation of an
  * N^3 complex spectrum in FFTW order, FP64 plain kernel.  Host arrays. */
 int edb_qcalc_calQ(edb_qcalc* q, double* qout, const double* fin);
+/* calQ with the FP32 copy of beta (edb_qcalc_genB_plain32). */
+int edb_qcalc_calQ_plain32(edb_qcalc* q, double* qout, const double* fin);
 
 /* Same on device arrays (FFTW-order N^3 complex). */
 int edb_qcalc_calQ_device(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream);
@@ -128,6 +135,17 @@ int edb_qcalc_tune(edb_qcalc* q, int batched_waves, int plain_slabs);
  * (batched at `batch` instances, Hermitian half) with HIP events and keeps
  * the fastest; returns them.  No reference counterpart. */
 int edb_qcalc_autotune(edb_qcalc* q, int batch, int* batched_waves, int* plain_slabs);
+
+/* Work schedule of the batched kernel.  EDB_SCHED_ROWS: stream-K over
+ * (k-row, instance group, l-row) steps with `count` persistent waves.
+ * EDB_SCHED_QUADS: workgroups of 4 neighbouring k-rows walking the union of
+ * their l-row schedules together, so the f rows they share come from L2;
+ * `count` persistent workgroups.  count 0 = built-in default.  The
+ * autotuner times both and keeps the faster.  No reference counterpart. */
+#define EDB_SCHED_ROWS 0
+#define EDB_SCHED_QUADS 1
+int edb_qcalc_tune_batched(edb_qcalc* q, int schedule, int count);
+int edb_qcalc_batched_schedule(const edb_qcalc* q, int* schedule, int* count);
 
 /* Conservative moment correction of every Q this handle returns (calQ,
  * calQ_device, calQ_batched, calQ_batched_device): the least-squares nearest

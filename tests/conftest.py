@@ -42,6 +42,15 @@ def rand_spectrum(N, seed, hermitian=False):
     return rng.standard_normal((N, N, N)) + 1j * rng.standard_normal((N, N, N))
 
 
+def hermitize(fh):
+    """Exact Hermitian part of a spectrum, (f_k + conj f_[-k]) / 2: fftn of real
+    data is Hermitian only to rounding, and the Hermitian-half kernel mirrors
+    Q_[-k] = conj Q_k exactly, so at modes where |Q| is at the rounding level of
+    the large ones the comparison needs the exact symmetry."""
+    rev = np.roll(np.flip(fh, axis=(-3, -2, -1)), 1, axis=(-3, -2, -1))
+    return 0.5 * (fh + np.conj(rev))
+
+
 def maxwellian_mixture(N, L, seed, ncomp=None):
     """Mixture of 2-4 isotropic Maxwellians (means U[-2,2]^3, T U[0.5,1.5],
     Dirichlet(1) weights), sampled on x_i = -L + i*2L/N and scaled to mass 1."""

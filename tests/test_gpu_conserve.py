@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import maxwellian_mixture, rand_spectrum
+from conftest import hermitize, maxwellian_mixture, rand_spectrum
 from test_oracle_conserve import moment_scale, moments
 
 pytestmark = pytest.mark.gpu
@@ -44,7 +44,7 @@ def test_plain_corrected_matches_oracle(edb, N, alpha):
 def test_batched_corrected_matches_oracle(edb, N, batch, hermitian):
     L = 8.0
     if hermitian:
-        fs = np.stack([np.fft.fftn(maxwellian_mixture(N, L, 80 + i)) for i in range(batch)])
+        fs = hermitize(np.stack([np.fft.fftn(maxwellian_mixture(N, L, 80 + i)) for i in range(batch)]))
     else:
         fs = np.stack([rand_spectrum(N, 90 + i) for i in range(batch)])
     q = edb.QCalculator(N, 0, L)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import maxwellian_mixture, rand_spectrum
+from conftest import hermitize, maxwellian_mixture, rand_spectrum
 
 pytestmark = pytest.mark.gpu
 
@@ -64,6 +64,36 @@ def test_plain_calQ_even_N(edb, N):
     assert rel(q.calQ(f), oracle.calQ(N, B, f)) < 1e-12
 
 
+@pytest.mark.parametrize("N,alpha", [(9, 0), (8, 1), (16, 0), (24, 0)])
+def test_plain32_calQ_matches_oracle(edb, N, alpha):
+    """FP32 copy of beta, FP64 products and sums: each term carries one FP32
+    rounding of beta, so the error is within 2^-24 * S_k (bar 1e-7 * S_k)."""
+    L = 8.0
+    f = np.fft.fftn(maxwellian_mixture(N, L, 170 + N))
+    q = edb.QCalculator(N, alpha, L)
+    q.closeFilter()
+    q.genB_plain32()
+    got = q.calQ_plain32(f)
+    B = oracle.genB(N, alpha, L, 0)
+    ref = oracle.calQ(N, B, f)
+    S = bound(N, B, f)
+    assert np.all(np.abs(got - ref) <= 1e-7 * S + 1e-300)
+    assert got[0, 0, 0] == 0.0  # beta(l, -l) is exactly 0 in FP32 too
+
+
+def test_solver_plain32_rk4_matches_oracle(edb):
+    N, L, dt = 16, 8.0, 0.01
+    F0 = _initial(N, L, 2, seed=50)
+    s = edb.Solver(N, 0, L, False, 2, edb.PREC_FP32_PLAIN, "RK4", dt, 0.0, edb.FIX_APPLIED, history=8)
+    s.set_initial_real(F0)
+    s.step(5)
+    got = s.get_spectral()
+    B = oracle.genB(N, 0, L, 0)
+    for i in range(2):
+        fo, *_ = oracle.run(N, B, L, "RK4", 0.0, 0.05 - 1e-12, dt, np.fft.fftn(F0[i]), fix_mode=2)
+        assert rel(got[i], fo) < 1e-7, i
+
+
 @pytest.mark.parametrize("N,batch", [(5, 1), (8, 3), (9, 70), (16, 2), (25, 1)])
 def test_batched_calQ_general_input(edb, N, batch):
     fs = np.stack([rand_spectrum(N, 7 * N + i) for i in range(batch)])
@@ -80,7 +110,7 @@ def test_batched_calQ_general_input(edb, N, batch):
 @pytest.mark.parametrize("N,batch", [(9, 4), (16, 65), (17, 2), (24, 2), (32, 1)])
 def test_batched_calQ_hermitian(edb, N, batch):
     L = 8.0
-    fs = np.stack([np.fft.fftn(maxwellian_mixture(N, L, 40 + i)) for i in range(batch)])
+    fs = hermitize(np.stack([np.fft.fftn(maxwellian_mixture(N, L, 40 + i)) for i in range(batch)]))
     q = edb.QCalculator(N, 0, L)
     q.closeFilter()
     q.genB_batched()
@@ -94,6 +124,34 @@ def test_batched_calQ_hermitian(edb, N, batch):
         assert got[i][0, 0, 0] == 0.0
     # the Hermitian half equals the full evaluation on real data
     assert np.abs(got - full).max() <= 3e-6 * np.abs(full).max()
+
+
+@pytest.mark.parametrize("N,batch,hermitian", [(9, 3, True), (16, 130, True), (24, 65, True),
+                                               (32, 64, True), (8, 5, False), (25, 2, True)])
+def test_batched_schedules_agree_with_oracle(edb, N, batch, hermitian):
+    """Stream-K rows and workgroup quads: same operator, different work split."""
+    L = 8.0
+    if hermitian:
+        fs = hermitize(np.stack([np.fft.fftn(maxwellian_mixture(N, L, 300 + i)) for i in range(batch)]))
+    else:
+        fs = np.stack([rand_spectrum(N, 310 + i) for i in range(batch)])
+    q = edb.QCalculator(N, 0, L)
+    q.closeFilter()
+    q.genB_batched()
+    out = {}
+    for sched, count in ((0, 0), (1, 0), (1, 7), (0, 1000)):
+        q.tune_batched(sched, count)
+        out[(sched, count)] = q.calQ_batched(fs, hermitian=hermitian)
+    B = oracle.genB(N, 0, L, 0)
+    for i in sorted({0, batch - 1}):
+        ref = oracle.calQ(N, B, fs[i])
+        S = bound(N, B, fs[i])
+        for key, got in out.items():
+            assert np.all(np.abs(got[i] - ref) <= 3e-6 * S + 1e-300), (key, i)
+            assert got[i][0, 0, 0] == 0.0
+    base = out[(0, 0)]
+    for key, got in out.items():  # only the FP64 merge order of partial sums differs
+        assert np.abs(got - base).max() <= 1e-12 * np.abs(base).max(), key
 
 
 def test_plain_and_batched_timers(edb):
