@@ -94,7 +94,7 @@ def test_solver_plain32_rk4_matches_oracle(edb):
         assert rel(got[i], fo) < 1e-7, i
 
 
-@pytest.mark.parametrize("N,batch", [(5, 1), (8, 3), (9, 70), (16, 2), (25, 1)])
+@pytest.mark.parametrize("N,batch", [(3, 2), (4, 1), (5, 1), (8, 3), (9, 70), (16, 2), (25, 1)])
 def test_batched_calQ_general_input(edb, N, batch):
     fs = np.stack([rand_spectrum(N, 7 * N + i) for i in range(batch)])
     q = edb.QCalculator(N, 1, 8.0)
@@ -152,6 +152,34 @@ def test_batched_schedules_agree_with_oracle(edb, N, batch, hermitian):
     base = out[(0, 0)]
     for key, got in out.items():  # only the FP64 merge order of partial sums differs
         assert np.abs(got - base).max() <= 1e-12 * np.abs(base).max(), key
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_largest_grid_batched_matches_plain(edb, schedule):
+    """N = 33, the largest compiled batched kernel: FP32 batched (both
+    schedules) against the FP64 plain path on the GPU (whose operator the
+    smaller-N tests pin to the oracle), Q_0 = 0 and exact Hermitian output."""
+    N, L, batch = 33, 8.0, 3
+    fs = hermitize(np.stack([np.fft.fftn(maxwellian_mixture(N, L, 400 + i)) for i in range(batch)]))
+    qb = edb.QCalculator(N, 1, L)
+    qb.closeFilter()
+    qb.genB_batched()
+    qb.tune_batched(schedule, 0)
+    got = qb.calQ_batched(fs, hermitian=True)
+    qp = edb.QCalculator(N, 1, L)
+    qp.closeFilter()
+    qp.genB()
+    for i in range(batch):
+        ref = qp.calQ(fs[i])
+        assert np.abs(got[i] - ref).max() <= 2e-6 * np.abs(ref).max(), i
+        assert got[i][0, 0, 0] == 0.0
+        # mirrored entries are exact conjugates; the k1 = k2 = 0 line is computed
+        # directly on both sides, so there they agree to rounding
+        rev = np.conj(np.roll(np.flip(got[i], axis=(0, 1, 2)), 1, axis=(0, 1, 2)))
+        off = np.ones((N, N, N), dtype=bool)
+        off[0, 0, :] = False
+        assert np.array_equal(got[i][off], rev[off])
+        assert np.abs(got[i][0, 0, :] - rev[0, 0, :]).max() <= 1e-6 * np.abs(ref).max()
 
 
 def test_plain_and_batched_timers(edb):
