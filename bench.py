@@ -141,7 +141,7 @@ def serve(edb, args):
     for i, ta in enumerate(arrivals):
         reqs.append((i, ta, int(rng.choice([16, 24, 32])), int(rng.integers(0, 2)),
                      float(rng.uniform(1.0, 5.0))))
-    srv = edb.Server(L=L, dt=dt, fix_mode=edb.FIX_APPLIED, max_chunk=8)
+    srv = edb.Server(L=L, dt=dt, fix_mode=edb.FIX_APPLIED, max_chunk=8, max_pending=args.max_pending)
     # warm the six groups (collision kernels built, buffers allocated) before the clock starts
     warm = 0
     for N in (16, 24, 32):
@@ -158,9 +158,10 @@ def serve(edb, args):
                 except edb.EdbError:
                     break
     steps_of = lambda te: int(np.ceil(te / dt - 1e-9))
-    done_t, lat, qevals = {}, [], 0
+    done_t, lat, qevals, rejected = {}, [], 0, 0
     t0 = time.perf_counter()
     nxt, backlog = 0, []
+    next_report = t0 + 20.0
     end_arrivals = t0 + args.duration
     deadline = end_arrivals + args.drain_max
     while True:
@@ -171,7 +172,9 @@ def serve(edb, args):
         still = []
         for (i, ta, N, g, te) in backlog:
             st = srv.submit(i, N, g, te, mixture(N, L, args.seed * 100003 + i))
-            if st != 0:
+            if st != 0 and args.max_pending > 0:
+                rejected += 1  # admission limit: the request is turned away
+            elif st != 0:
                 still.append((i, ta, N, g, te))
         backlog = still
         if srv.poll():
@@ -184,8 +187,12 @@ def serve(edb, args):
                 lat.append(tc - (t0 + reqs[rid][1]))
                 qevals += 4 * steps
                 done_t[rid] = tc
-        if nxt >= len(reqs) and len(done_t) == len(reqs):
+        if nxt >= len(reqs) and len(done_t) + rejected == len(reqs):
             break
+        if now >= next_report:  # progress of a long run, on stderr
+            print(f"serve: t={now - t0:.0f}s arrived {nxt} completed {len(done_t)} rejected {rejected}",
+                  file=sys.stderr, flush=True)
+            next_report += 20.0
         if now > deadline:
             break
     wall = time.perf_counter() - t0
@@ -203,11 +210,17 @@ def serve(edb, args):
     seq = sum(4 * steps_of(te) * plain[N] for (_i, _ta, N, _g, te) in reqs)
     out = {"metric": "serving: p50/p99 solve latency, throughput",
            "rate_req_s": args.rate, "duration_s": args.duration, "requests": len(reqs),
-           "completed": int(len(lat)), "unfinished": int(len(reqs) - len(lat)),
+           "completed": int(len(lat)), "rejected": int(rejected), "max_pending": args.max_pending,
+           "unfinished": int(len(reqs) - len(lat) - rejected),
            "wall_s": round(wall, 2),
            "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
            "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+           "p90_s": float(np.percentile(lat, 90)) if len(lat) else None,
            "throughput_req_s": round(len(lat) / wall, 2), "qevals_per_s": round(qevals / wall, 1),
+           # completions in the second half of the arrival window: the rate the device sustains
+           "throughput_steady_req_s": round(
+               sum(1 for tc in done_t.values() if t0 + args.duration / 2 <= tc <= end_arrivals)
+               / (args.duration / 2), 2),
            "one_at_a_time_fp64_req_s": round(len(reqs) / seq, 3),
            "one_at_a_time_note": "plain FP64 Q-evaluation time per N (autotuned) x 4 Q-evals per RK4 "
                                  "step of each request, summed over this request mix",
@@ -482,6 +495,8 @@ def main():
     ap.add_argument("--duration", type=float, default=10.0, help="seconds of arrivals (--serve)")
     ap.add_argument("--drain-max", type=float, default=60.0,
                     help="seconds allowed after the arrivals stop (--serve)")
+    ap.add_argument("--max-pending", type=int, default=0,
+                    help="admission limit of the server (--serve): requests beyond it are rejected")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--experiment", choices=["bkw16", "bkw24", "n32", "check256"],
                     help="run one of the reference configurations with its error report")

@@ -330,11 +330,12 @@ class Server:
     """Continuous batching of independent RK4 solves (include/edb.h, edb_server_*)."""
 
     def __init__(self, L=8.0, filter=False, dt=0.01, fix_mode=FIX_APPLIED, max_slots=0, max_chunk=0,
-                 hbm_budget=0, conserve=False):
+                 hbm_budget=0, conserve=False, max_pending=0):
         p = _lib.ServerParams()
         p.L, p.filter, p.dt, p.fix_mode = float(L), int(bool(filter)), float(dt), int(fix_mode)
         p.max_slots, p.max_chunk, p.hbm_budget = int(max_slots), int(max_chunk), int(hbm_budget)
         p.conserve = int(bool(conserve))
+        p.max_pending = int(max_pending)
         self.p = p
         self._h = ctypes.c_void_p()
         check(load().edb_server_create(ctypes.byref(p), ctypes.byref(self._h)), "edb_server_create")

@@ -50,7 +50,8 @@ class SolverParams(ctypes.Structure):
 class ServerParams(ctypes.Structure):
     _fields_ = [("L", ctypes.c_double), ("filter", ctypes.c_int), ("dt", ctypes.c_double),
                 ("fix_mode", ctypes.c_int), ("max_slots", ctypes.c_int), ("max_chunk", ctypes.c_int),
-                ("hbm_budget", ctypes.c_long), ("conserve", ctypes.c_int)]
+                ("hbm_budget", ctypes.c_long), ("conserve", ctypes.c_int),
+                ("max_pending", ctypes.c_int)]
 
 
 _SIGS = {

@@ -293,6 +293,7 @@ int edb_server_create(const edb_server_params* p, edb_server** out) {
               "server_create: fix_mode must be NONE or APPLIED");
   EDB_REQUIRE(p->max_slots >= 0 && p->max_slots <= 4096, "server_create: max_slots out of range");
   EDB_REQUIRE(p->max_chunk >= 0, "server_create: max_chunk must be >= 0");
+  EDB_REQUIRE(p->max_pending >= 0, "server_create: max_pending must be >= 0");
   edb_server* s = new edb_server();
   s->p = *p;
   if (s->p.max_chunk == 0) s->p.max_chunk = 16;
@@ -305,6 +306,14 @@ int edb_server_submit(edb_server* s, long id, int N, int alpha, double t_end, co
   EDB_REQUIRE(N >= 3 && N <= kMaxNCalq && batched_supported(N), "server_submit: unsupported N");
   EDB_REQUIRE(alpha == 0 || alpha == 1, "server_submit: alpha must be 0 or 1");
   EDB_REQUIRE(t_end > 0. && std::isfinite(t_end), "server_submit: t_end must be positive");
+  if (s->p.max_pending > 0) {
+    long waiting = 0;
+    for (auto& kv : s->groups) waiting += (long)kv.second->pending.size();
+    if (waiting >= s->p.max_pending) {
+      set_error("server_submit: admission limit reached (" + std::to_string(waiting) + " waiting); retry later");
+      return EDB_EBUSY;
+    }
+  }
   auto key = std::make_pair(N, alpha);
   if (!s->groups.count(key)) {
     int cap = pad64(s->p.max_slots > 0 ? s->p.max_slots : (N >= 27 ? 64 : 256));

@@ -278,6 +278,8 @@ typedef struct {
   int max_chunk;     /* RK4 steps per launch round (0: 16) */
   long hbm_budget;   /* bytes for kernels + slots over all groups (0: unlimited) */
   int conserve;      /* moment correction of every Q (edb_qcalc_set_conservative) */
+  int max_pending;   /* admission limit: submit returns EDB_EBUSY while this many
+                        requests wait for a slot (0: unlimited) */
 } edb_server_params;
 
 typedef struct edb_server edb_server;

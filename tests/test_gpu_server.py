@@ -77,3 +77,16 @@ def test_server_hbm_budget(edb):
     assert srv.submit(2, 9, 1, 0.05, F1) == 0  # idle group evicted
     out = drain(srv, 1)
     assert 2 in out
+
+
+def test_server_admission_limit(edb):
+    """max_pending turns requests away (EDB_EBUSY) while that many wait, and
+    admits again once the backlog drains."""
+    srv = edb.Server(L=L, dt=DT, fix_mode=edb.FIX_APPLIED, max_slots=64, max_chunk=2, max_pending=3)
+    F0 = maxwellian_mixture(9, L, 900)
+    st = [srv.submit(i, 9, 0, 0.02, F0) for i in range(5)]
+    assert st[:3] == [0, 0, 0] and st[3] != 0 and st[4] != 0
+    done = drain(srv, 3)
+    assert sorted(done) == [0, 1, 2]
+    assert srv.submit(10, 9, 0, 0.02, F0) == 0
+    assert 10 in drain(srv, 1)
