@@ -248,29 +248,59 @@ __device__ inline int wave_of(long total, int nwaves, long x) {
 // waves splitting the k3 entries.  Items with no interior boundary were
 // written by their wave.
 __global__ __launch_bounds__(256) void calq_fixup_kernel(BatchedArgs A, int N, int KS) {
+  constexpr int kMaxCover = 64;
+  __shared__ long soff[kMaxCover];  // slot base of each covering wave, in double2 units
   const int n = (N - 1) / 2, M = 2 * n + 1, KPT = (M + KS - 1) / KS;
   const int wb = blockIdx.x + 1;
-  const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
   if (wb >= A.nwaves) return;
   const long p = range_start(A.total, A.nwaves, wb);
   const int it = A.wstart[wb];
   const long lo = A.prefix[it], hi = A.prefix[it + 1];
   if (lo == p || range_start(A.total, A.nwaves, wb - 1) > lo) return;
   const int wf = wave_of(A.total, A.nwaves, lo), wl = wave_of(A.total, A.nwaves, hi - 1);
+  const int nw = wl - wf + 1;
+  for (int i = threadIdx.x; i < nw && i < kMaxCover; i += blockDim.x) {
+    const int w = wf + i;
+    const int slot = range_start(A.total, A.nwaves, w) >= lo ? 0 : 1;
+    soff[i] = (long)(2 * w + slot) * KPT * 64;
+  }
+  __syncthreads();
   const int r = it / (KS * A.groups), h = (it / A.groups) % KS, g = it % A.groups;
   const int4 ri = A.rowinfo[r];
   const bool mirror = A.hermitian && !(ri.x == n && ri.y == n);
-  const long col = (long)g * 64 + lane;
-  for (int j = sub; j < KPT; j += 4) {
+  // one thread per (k3 entry, lane); the covering waves' loads are issued together
+  for (int e = threadIdx.x; e < KPT * 64; e += blockDim.x) {
+    const int j = e >> 6, lane = e & 63;
     const int k3 = h * KPT + j;
-    if (k3 >= M) break;
+    if (k3 >= M) continue;
+    const long rel = (long)j * 64 + lane;
     double sr = 0., si = 0.;
-    for (int w = wf; w <= wl; ++w) {
-      const int slot = range_start(A.total, A.nwaves, w) >= lo ? 0 : 1;
-      const double2 v = A.slots[((long)(2 * w + slot) * KPT + j) * 64 + lane];
+    int i = 0;
+    for (; i + 4 <= nw && i + 4 <= kMaxCover; i += 4) {
+      const double2 v0 = A.slots[soff[i] + rel], v1 = A.slots[soff[i + 1] + rel];
+      const double2 v2 = A.slots[soff[i + 2] + rel], v3 = A.slots[soff[i + 3] + rel];
+      sr += v0.x;
+      si += v0.y;
+      sr += v1.x;
+      si += v1.y;
+      sr += v2.x;
+      si += v2.y;
+      sr += v3.x;
+      si += v3.y;
+    }
+    for (; i < nw; ++i) {  // beyond the table: recompute the slot (long items only)
+      long base;
+      if (i < kMaxCover) {
+        base = soff[i];
+      } else {
+        const int w = wf + i;
+        base = (long)(2 * w + (range_start(A.total, A.nwaves, w) >= lo ? 0 : 1)) * KPT * 64;
+      }
+      const double2 v = A.slots[base + rel];
       sr += v.x;
       si += v.y;
     }
+    const long col = (long)g * 64 + lane;
     A.q[(((long)ri.x * M + ri.y) * M + k3) * A.Bp + col] = make_double2(sr, si);
     if (mirror)
       A.q[(((long)(M - 1 - ri.x) * M + (M - 1 - ri.y)) * M + (M - 1 - k3)) * A.Bp + col] =
@@ -402,32 +432,61 @@ __global__ __launch_bounds__(64 * kQuad * KS) __attribute__((amdgpu_waves_per_eu
 // boundary splits is summed, per wave (k-row j, k3 part h), over the slots of
 // the blocks that cover it, by the block of the first such boundary.
 __global__ __launch_bounds__(256 * 2) void calq_quad_fixup_kernel(QuadArgs A, int N, int KS) {
-  const int n = (N - 1) / 2, M = 2 * n + 1, KPT = (M + KS - 1) / KS;
+  constexpr int kMaxCover = 64;
+  __shared__ long soff[kMaxCover];  // slot base of each covering block, in double2 units
+  const int n = (N - 1) / 2, M = 2 * n + 1, KPT = (M + KS - 1) / KS, WPB = kQuad * KS;
   const int wb = blockIdx.x + 1;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j = wave % kQuad, h = wave / kQuad;
-  if (wb >= A.nblocks || wave >= kQuad * KS) return;
+  if (wb >= A.nblocks) return;
   const long p = range_start(A.total, A.nblocks, wb);
   const int it = A.bstart[wb];
   const long lo = A.prefix[it], hi = A.prefix[it + 1];
   if (lo == p || range_start(A.total, A.nblocks, wb - 1) > lo) return;
   const int bf = wave_of(A.total, A.nblocks, lo), bl = wave_of(A.total, A.nblocks, hi - 1);
+  const int nb = bl - bf + 1;
+  for (int i = threadIdx.x; i < nb && i < kMaxCover; i += blockDim.x) {
+    const int b = bf + i;
+    const int slot = range_start(A.total, A.nblocks, b) >= lo ? 0 : 1;
+    soff[i] = (long)(2 * b + slot) * WPB * KPT * 64;
+  }
+  __syncthreads();
   const int qd = it / A.groups, g = it % A.groups;
   const int nrows = A.quadinfo[qd].z;
-  if (j >= nrows) return;
-  const int4 ri = A.rowinfo[A.rowidx[qd * kQuad + j]];
-  const bool mirror = A.hermitian && !(ri.x == n && ri.y == n);
-  const long col = (long)g * 64 + lane;
-  for (int t = 0; t < KPT; ++t) {
+  // one thread per (wave of the block, k3 entry, lane)
+  for (int e = threadIdx.x; e < WPB * KPT * 64; e += blockDim.x) {
+    const int wv = e / (KPT * 64), t = (e / 64) % KPT, lane = e & 63;
+    const int j = wv % kQuad, h = wv / kQuad;
     const int k3 = h * KPT + t;
-    if (k3 >= M) break;
+    if (j >= nrows || k3 >= M) continue;
+    const long rel = (long)wv * KPT * 64 + (long)t * 64 + lane;
     double sr = 0., si = 0.;
-    for (int b = bf; b <= bl; ++b) {
-      const int slot = range_start(A.total, A.nblocks, b) >= lo ? 0 : 1;
-      const double2 v = A.slots[((long)(2 * b + slot) * (kQuad * KS) + wave) * KPT * 64 + (long)t * 64 + lane];
+    int i = 0;
+    for (; i + 4 <= nb && i + 4 <= kMaxCover; i += 4) {
+      const double2 v0 = A.slots[soff[i] + rel], v1 = A.slots[soff[i + 1] + rel];
+      const double2 v2 = A.slots[soff[i + 2] + rel], v3 = A.slots[soff[i + 3] + rel];
+      sr += v0.x;
+      si += v0.y;
+      sr += v1.x;
+      si += v1.y;
+      sr += v2.x;
+      si += v2.y;
+      sr += v3.x;
+      si += v3.y;
+    }
+    for (; i < nb; ++i) {
+      long base;
+      if (i < kMaxCover) {
+        base = soff[i];
+      } else {
+        const int b = bf + i;
+        base = (long)(2 * b + (range_start(A.total, A.nblocks, b) >= lo ? 0 : 1)) * WPB * KPT * 64;
+      }
+      const double2 v = A.slots[base + rel];
       sr += v.x;
       si += v.y;
     }
+    const int4 ri = A.rowinfo[A.rowidx[qd * kQuad + j]];
+    const bool mirror = A.hermitian && !(ri.x == n && ri.y == n);
+    const long col = (long)g * 64 + lane;
     A.q[(((long)ri.x * M + ri.y) * M + k3) * A.Bp + col] = make_double2(sr, si);
     if (mirror)
       A.q[(((long)(M - 1 - ri.x) * M + (M - 1 - ri.y)) * M + (M - 1 - k3)) * A.Bp + col] =

@@ -70,7 +70,8 @@ int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out) {
 int ensure_slots(edb_qcalc* q, int nwaves) {
   size_t need = batched_slot_doubles(q->N, nwaves) + (size_t)(nwaves + 1) / 2;
   if (need <= q->slots_count) return kOk;
-  dfree(&q->d_slots);
+  if (q->d_slots) q->retired.push_back(q->d_slots);
+  q->d_slots = nullptr;
   q->slots_count = 0;
   EDB_TRY(dmalloc(&q->d_slots, need));
   q->slots_count = need;
@@ -475,8 +476,10 @@ int edb_qcalc_tune(edb_qcalc* q, int batched_waves, int plain_slabs) {
   if (plain_slabs > 0) {
     int ns = (int)std::min<long>(plain_slabs, q->g.NQ);
     if (ns != q->nslab) {
-      dfree(&q->d_part);
-      dfree(&q->d_dup);
+      if (q->d_part) q->retired.push_back(q->d_part);
+      if (q->d_dup) q->retired.push_back(q->d_dup);
+      q->d_part = nullptr;
+      q->d_dup = nullptr;
       q->nslab = ns;
     }
   }
@@ -579,6 +582,7 @@ int edb_qcalc_destroy(edb_qcalc* q) {
   dfree(&q->d_ublk);
   dfree(&q->d_slots);
   for (auto& kv : q->prefixes) dfree(&kv.second.d_prefix);
+  for (void* p : q->retired) (void)hipFree(p);
   if (q->tables_ready) beta_tables_free(&q->tables);
   if (q->stream) (void)hipStreamDestroy(q->stream);
   delete q;

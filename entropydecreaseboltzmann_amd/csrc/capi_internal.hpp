@@ -77,6 +77,9 @@ struct edb_qcalc {
   int4* d_quadinfo = nullptr;
   int* d_rowidx = nullptr;
   int* d_ublk = nullptr;
+  // device buffers replaced by larger ones; kept until destroy because a HIP
+  // graph captured earlier may still name them
+  std::vector<void*> retired;
   // conservative moment correction
   int conserve = 0;
   double* d_mom = nullptr;
