@@ -231,8 +231,8 @@ def serve(edb, args):
 
 # ------------------------------------------------------------------ experiments
 # The reference's own configurations, run end to end: errors against the
-# exact BKW solution and against the FP64 CPU oracle (used here only as the
-# checker), conservation and the H-theorem.  One JSON line each.
+# exact BKW solution and against the FP64 CPU oracle (used here only to check
+# the results), conservation and the H-theorem.  One JSON line each.
 
 def bkw_grid(N, L, K):
     """BKW profile f(K, v) on x_i = -L + i*2L/N (testfft.cpp:11-16 form)."""

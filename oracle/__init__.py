@@ -4,7 +4,8 @@ ctypes face of ``oracle/liboracle.so`` (a plain-C restatement of the reference,
 see edb_oracle.c) and, when it was built in this checkout, of
 ``oracle/_ref/libedbref.so`` (the reference's own QCalculator.h / EntropyFix.h
 compiled from their sources).  Only tests/, ``__graft_entry__.smoke()`` and
-``bench.py``'s CPU-baseline leg import this module, and only as the checker;
+``bench.py``'s CPU-baseline leg import this module, and only to check results against it
+(or to time the CPU baseline);
 the product (entropydecreaseboltzmann_amd) never does.
 """
 import ctypes

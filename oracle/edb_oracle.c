@@ -2,8 +2,8 @@
  * edb_oracle.c -- CPU oracle (test infrastructure) for the CalQ path.
  *
  * TEST INFRASTRUCTURE ONLY: a plain-C restatement of the reference
- * algorithm, used by tests/, smoke() and bench.py's cpu_baseline leg as the
- * checker.  It is never linked into or called by the HIP library.
+ * algorithm, used by tests/, smoke() and bench.py's cpu_baseline leg to check
+ * the HIP library's results.  It is never linked into or called by the HIP library.
  *
  * Every function cites the reference lines it restates (paths relative to
  * the reference repository).  Arithmetic is written operation by operation in

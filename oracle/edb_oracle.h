@@ -2,8 +2,8 @@
  * edb_oracle.h -- CPU oracle for the CalQ collision-operator path.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
- * algorithm (YK2015/EntropyDecreaseBoltzmann, the CalQ headers) used as the checker
- * for the HIP library.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * algorithm (YK2015/EntropyDecreaseBoltzmann, the CalQ headers) that the HIP
+ * library's results are checked against.  Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it, and only to check or to time the CPU
  * baseline.  The product path never calls into it.
  *
