@@ -7,6 +7,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -24,9 +25,35 @@ static void dump(const std::string& path, const void* p, size_t bytes) {
   std::fclose(f);
 }
 
+// The six drivers with the reference's per-step output (BKW3D::setOutput),
+// each table written to <dir>/table_<scheme>.txt: N = 11, R = 6, BKW data at
+// s = 0.8 (T0 = 6 ln 2 for s(t) = 1 - 0.4 exp(-t/6)), dt = 0.05 to T0 + 0.4,
+// beta = 2 for CFE and DUFORT, fix on.
+static int driver_tables(const std::string& dir) {
+  const double T0 = 6. * std::log(2.), dt = 0.05, TEnd = T0 + 0.4;
+  const char* names[] = {"FEuler", "RK2", "SSP3", "RK4", "CFE", "DUFORT"};
+  for (int sc = 0; sc < 6; ++sc) {
+    std::ofstream out(dir + "/table_" + names[sc] + ".txt");
+    edb::BKW3D<c_type, r_type> bkw(11, 6.0);
+    bkw.setFixEntropy(true);
+    bkw.setOutput(&out, false);
+    switch (sc) {
+      case 0: bkw.FEuler(T0, TEnd, dt); break;
+      case 1: bkw.RK2(T0, TEnd, dt); break;
+      case 2: bkw.SSP3(T0, TEnd, dt); break;
+      case 3: bkw.RK4(T0, TEnd, dt); break;
+      case 4: bkw.CFE(T0, TEnd, dt, 2.0); break;
+      default: bkw.DUFORT(T0, TEnd, dt, 2.0); break;
+    }
+    dump(dir + "/table_F_" + names[sc] + ".bin", bkw.F().data(), sizeof(double) * bkw.F().size());
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) return 2;
   std::string dir = argv[1];
+  if (argc > 2 && std::string(argv[2]) == "tables") return driver_tables(dir);
   // CalQ/test.cpp: genB + calQ on inputs rand()%10 (here a fixed LCG).
   int N0 = 9, N = N0 * N0 * N0;
   std::vector<c_type> fin(N), Qout(N);

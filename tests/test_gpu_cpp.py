@@ -50,3 +50,83 @@ def test_cpp_mirror_matches_oracle(tmp_path, edb):
         assert abs(Fb.mean() - ref_F.mean()) <= 1e-12 * abs(ref_F.mean())
         assert oracle.cal_entropy(N, 8.0, Fb.astype(np.complex128)) == pytest.approx(Hf[-1], rel=1e-9)
     np.testing.assert_allclose(ld("bkw_H.bin"), Hf, rtol=1e-9)
+
+
+SCHEMES = ["FEuler", "RK2", "SSP3", "RK4", "CFE", "DUFORT"]
+
+
+def _table_rows(path):
+    """Data rows of one driver table (lines that are not '#' comments), split on tabs."""
+    rows, sections = [], []
+    for line in open(path).read().splitlines():
+        if line.startswith("# Start the evolution using"):
+            sections.append(line)
+        elif not line.startswith("#"):
+            rows.append(line.split("\t"))
+    return rows, sections
+
+
+def test_cpp_driver_tables_match_oracle(tmp_path, edb):
+    """BKW3D::setOutput: each driver's progress lines and per-step table in the
+    reference's format (BKW.h:212-633), every printed number against the
+    oracle's driver (diagnostic fix, which fires on this data)."""
+    r = subprocess.run([EXE, str(tmp_path), "tables"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    N, R, dt = 11, 6.0, 0.05
+    T0 = 6 * np.log(2.0)
+    F0 = oracle.grid_bkw(N, R, 0.8).real
+    B = oracle.genB(N, 0, R, False)
+    mean0 = F0.mean()
+    fixes = 0
+    for sc in SCHEMES:
+        rows, sections = _table_rows(tmp_path / f"table_{sc}.txt")
+        fo, t, H, Hf = oracle.run(N, B, R, sc, T0, T0 + 0.4, dt, np.fft.fftn(F0), beta=2.0, fix_mode=1)
+        name = {"FEuler": "Forward Euler", "CFE": "C-Forward Euler"}.get(sc, sc)
+        if sc == "DUFORT":
+            assert sections == ["# Start the evolution using RK2...", "# Start the evolution using DUFORT..."]
+        else:
+            assert sections == [f"# Start the evolution using {name}..."]
+        fixed = (Hf != H) & np.isfinite(H)
+        fixes += int(fixed.sum())
+        if sc == "FEuler":  # row 00: t0 and calEntropy of the initial F
+            assert rows[0][0] == "00" and float(rows[0][2]) == pytest.approx(
+                oracle.cal_entropy(N, R, F0.astype(np.complex128)), abs=2e-8)
+            prev = float(rows[0][2])
+            rows = rows[1:]
+        else:
+            prev = 1.0
+        assert len(rows) == len(t), sc
+        for k, row in enumerate(rows):
+            kk = k - 8 if (sc == "DUFORT" and k >= 8) else k  # DUFORT restarts its count
+            if sc == "FEuler":
+                assert int(row[0]) == k + 1
+                row = row[1:]
+            assert float(row[0]) == pytest.approx(t[k], abs=1e-8)
+            if sc in ("RK2", "DUFORT", "CFE") and not (sc == "CFE" and kk == 0):
+                m = float(row[1])
+                if sc != "DUFORT":
+                    assert m == pytest.approx(mean0, abs=2e-8)  # Q_0 = 0 keeps the mean
+                row = [row[0]] + row[2:]
+            assert float(row[1]) == pytest.approx(H[k], abs=2e-8), (sc, k)
+            first = (sc == "FEuler" and k == 0) or (sc != "FEuler" and kk == 0)
+            if first:
+                assert row[2] == "-"
+                prev = Hf[k]  # entropy0 = entropy after every row
+                continue
+            assert float(row[2]) == pytest.approx(prev - H[k], abs=2e-8), (sc, k)
+            if fixed[k]:
+                assert float(row[3]) == pytest.approx(Hf[k], abs=2e-8)
+                assert float(row[4]) == pytest.approx(prev - Hf[k], abs=2e-8)
+            elif sc == "FEuler":
+                assert float(row[3]) == pytest.approx(H[k], abs=2e-8)
+            else:
+                assert row[3:] in ([], [""]), (sc, k, row)
+            prev = Hf[k]
+        # the driver's F after the last step: the fixed values if the fix ran
+        Fb = np.fromfile(tmp_path / f"table_F_{sc}.bin").reshape(N, N, N)
+        ref = np.fft.ifftn(fo).real
+        if fixed[-1]:
+            assert oracle.cal_entropy(N, R, Fb.astype(np.complex128)) == pytest.approx(Hf[-1], rel=1e-9)
+        else:
+            assert np.abs(Fb - ref).max() <= 1e-12 * np.abs(ref).max(), sc
+    assert fixes >= 3  # FEuler, SSP3, RK4 and CFE each fix once on this data

@@ -297,3 +297,39 @@ def test_instance_same_alone_and_in_mix(edb):
     alone.set_initial_real(F0[37:38])
     alone.step(3)
     assert rel(alone.get_spectral()[0], mix.get_spectral()[37]) < 1e-12
+
+
+def test_driver_real_and_means_with_diagnostic_fix(edb):
+    """In the diagnostic mode the driver's real-space F holds the fixed values
+    (the state does not): F_fix = (1 - b) F + b * avg with the EntropyFix
+    average (the mean of F at the first fix), b > 0, whose calEntropy is the
+    recorded Hfix; the mean history records mean(F) before and after the fix.
+    N = 11, R = 6, BKW data at K = 0.8: the RK4 fix fires once (its blend is
+    tiny, b ~ 1e-11, so the check is on that structure, not elementwise)."""
+    N, R, dt = 11, 6.0, 0.05
+    F0 = oracle.grid_bkw(N, R, 0.8)
+    B = oracle.genB(N, 0, R, 0)
+    fo, t, H, Hf = oracle.run(N, B, R, "RK4", 0.0, 0.4, dt, np.fft.fftn(F0), fix_mode=1)
+    fixed = np.flatnonzero((Hf != H) & np.isfinite(H))
+    assert fixed.size >= 1
+    k = int(fixed[0])
+    s = edb.Solver(N, 0, R, False, 1, edb.PREC_FP64_PLAIN, "RK4", dt, 0.0, edb.FIX_DIAGNOSTIC, history=16)
+    s.set_initial_real(F0[None])
+    s.step(k + 1)
+    Hg, Hfg, fxg = s.entropy_history(k + 1)
+    assert fxg[k, 0] == 1 and fxg[:k, 0].sum() == 0
+    got = s.get_driver_real()[0].ravel()
+    F = s.get_real()[0].ravel()  # the state, untouched by the diagnostic fix
+    fk, *_ = oracle.run(N, B, R, "RK4", 0.0, (k + 0.5) * dt, dt, np.fft.fftn(F0), fix_mode=1)
+    assert rel(F, np.fft.ifftn(fk).real.ravel()) < 1e-12
+    avg = F.mean()  # first fix: the average is set from this F (EntropyFix.h:225)
+    d, u = got - F, avg - F
+    b = np.dot(d, u) / np.dot(u, u)
+    assert 0 < b < 1e-6
+    assert np.abs(d - b * u).max() <= 1e-3 * np.abs(d).max() + 1e-30
+    assert oracle.cal_entropy(N, R, got.reshape(N, N, N).astype(np.complex128)) == pytest.approx(
+        Hfg[k, 0], rel=1e-12)
+    m, mf = s.mean_history(k + 1)
+    assert m[k, 0] == pytest.approx(F0.mean(), rel=1e-13)
+    assert mf[k, 0] == pytest.approx(got.mean(), rel=1e-13)
+    assert np.all(mf[:k, 0] == m[:k, 0])  # no fix before step k
