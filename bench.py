@@ -582,8 +582,13 @@ def main():
                      "flops_per_launch": flops_launch,
                      "flops_note": "canonical 10*M^6 per Q-eval (M = 23 modes per dim at N = 24); "
                                    "the kernel executes ~1/4 of the terms (Hermitian half x "
-                                   "(l,m) pairing), so frac can exceed 1",
+                                   "(l,m) pairing), so frac can exceed 1; executed_frac = the "
+                                   "executed flops (4 FMA + 2 MUL per pair-term) over the same peak; "
+                                   "issue_frac = its 6 VALU instructions per pair-term over the "
+                                   "dual-issue FP32 VALU rate (1024 SIMDs x 2.4 GHz x 2 per 4 cycles)",
                      "executed_tflops": round(executed / (qms * 1e-3) / 1e12, 2),
+                     "executed_frac": round(executed / (qms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                     "issue_frac": round(executed * 0.6 / 64 / (qms * 1e-3) / (1024 * 2.4e9 / 2), 4),
                      "batched_waves": tuned["batched_waves"]},
         "mass_drift_max": float(np.abs(mom[:, 0] - 1.0).max()),
     }
