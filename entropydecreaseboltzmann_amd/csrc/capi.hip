@@ -98,7 +98,7 @@ int batched_shape(edb_qcalc* q, int groups, int hermitian, PrefixEntry** pe, int
     EDB_TRY(ensure_slots(q, nb * wpb));
     *count = nb;
   } else {
-    int nwaves = q->nwaves > 0 ? q->nwaves : cu_count() * 8;
+    int nwaves = q->nwaves > 0 ? q->nwaves : cu_count() * 4 * batched_waves_per_simd(q->N);
     if (nwaves > (*pe)->total) nwaves = (int)(*pe)->total;
     EDB_TRY(ensure_slots(q, nwaves));
     *count = nwaves;

@@ -86,6 +86,8 @@ int launch_bkw_error(int N, int batch, int ld, double L, double K, const double*
 
 // calq_batched.hip
 int batched_ksplit(int N);
+// Waves per SIMD the batched kernels are built for at grid size N.
+int batched_waves_per_simd(int N);
 size_t batched_slot_doubles(int N, int nwaves);
 
 // Conservative moment correction (conserve.hip): table of the 3M-2 axis
