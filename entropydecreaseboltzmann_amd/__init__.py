@@ -20,11 +20,12 @@ import numpy as np
 
 from . import _lib
 from ._lib import (FIX_APPLIED, FIX_DIAGNOSTIC, FIX_NONE, PREC_FP32_BATCHED, PREC_FP32_PLAIN,
-                   PREC_FP64_PLAIN, SCHEMES, EdbError, check, load)
+                   PREC_FP64_COMPRESSED, PREC_FP64_PLAIN, SCHEMES, EdbError, check, load)
 
 __all__ = ["QCalculator", "fft3d", "EntropyFix", "Solver", "Server", "EdbError", "load",
            "device_count", "nmodes", "SCHEMES", "FIX_NONE", "FIX_DIAGNOSTIC",
-           "FIX_APPLIED", "PREC_FP64_PLAIN", "PREC_FP32_BATCHED", "PREC_FP32_PLAIN"]
+           "FIX_APPLIED", "PREC_FP64_PLAIN", "PREC_FP32_BATCHED", "PREC_FP32_PLAIN",
+           "PREC_FP64_COMPRESSED"]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -89,6 +90,31 @@ class QCalculator:
         check(load().edb_qcalc_calQ_plain32(self._h, _ptr(q.view(np.float64)), _ptr(f.view(np.float64))),
               "calQ_plain32")
         return q
+
+    def genB_compressed(self):
+        """Compressed beta (F tabulated over |l+m|^2, |l-m|^2, plus per-mode
+        sigma and Bmm) for the plain FP64 path; no M^6 array."""
+        check(load().edb_qcalc_genB_compressed(self._h), "genB_compressed")
+
+    def compressed_bytes(self):
+        b = ctypes.c_long()
+        check(load().edb_qcalc_compressed_bytes(self._h, ctypes.byref(b)), "compressed_bytes")
+        return b.value
+
+    def calQ_compressed(self, fin):
+        """calQ with beta formed on the fly from the compressed tables: equal to
+        calQ bit for bit."""
+        N = self.N
+        f = _c128(fin, (N, N, N))
+        q = np.empty_like(f)
+        check(load().edb_qcalc_calQ_compressed(self._h, _ptr(q.view(np.float64)), _ptr(f.view(np.float64))),
+              "calQ_compressed")
+        return q
+
+    def time_compressed(self, reps=5):
+        ms = ctypes.c_double()
+        check(load().edb_qcalc_time_compressed(self._h, int(reps), ctypes.byref(ms)), "time_compressed")
+        return ms.value
 
     def calQ_batched(self, fin, hermitian=False):
         N = self.N

@@ -22,6 +22,7 @@ EDB_EBUSY = 6
 PREC_FP64_PLAIN = 0
 PREC_FP32_BATCHED = 1
 PREC_FP32_PLAIN = 2
+PREC_FP64_COMPRESSED = 3
 
 SCHEMES = {"FEuler": 0, "RK2": 1, "SSP3": 2, "RK4": 3, "CFE": 4, "DUFORT": 5}
 FIX_NONE, FIX_DIAGNOSTIC, FIX_APPLIED = 0, 1, 2
@@ -79,6 +80,11 @@ _SIGS = {
     "edb_qcalc_tune_batched": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "edb_qcalc_batched_schedule": (ctypes.c_int, [_vp, _ip, _ip]),
     "edb_qcalc_calQ_plain32": (ctypes.c_int, [_vp, _dp, _dp]),
+    "edb_qcalc_genB_compressed": (ctypes.c_int, [_vp]),
+    "edb_qcalc_compressed_bytes": (ctypes.c_int, [_vp, _lp]),
+    "edb_qcalc_calQ_compressed": (ctypes.c_int, [_vp, _dp, _dp]),
+    "edb_qcalc_calQ_compressed_device": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "edb_qcalc_time_compressed": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "edb_solver_mean_history": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "edb_solver_get_driver_real": (ctypes.c_int, [_vp, _dp]),
     "edb_qcalc_destroy": (ctypes.c_int, [_vp]),

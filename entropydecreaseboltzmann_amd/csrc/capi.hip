@@ -135,14 +135,18 @@ int ensure_plain_scratch(edb_qcalc* q) {
 // Plain FP64 Q of one column: d_fc FP64 centred [M^3] with stride fc_stride
 // (complex elements), result centred into d_q with stride q_stride.
 int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, int q_stride,
-                 hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, bool fp32) {
-  if (fp32) EDB_REQUIRE(q->d_plain32, "calQ_plain32: genB_plain32 has not run");
+                 hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, int src) {
+  if (src == kPlainFP32) EDB_REQUIRE(q->d_plain32, "calQ_plain32: genB_plain32 has not run");
+  else if (src == kPlainCompressed) EDB_REQUIRE(q->d_ftab, "calQ_compressed: genB_compressed has not run");
   else EDB_REQUIRE(q->d_plain, "calQ: genB has not run");
   EDB_TRY(ensure_plain_scratch(q));
   EDB_TRY(launch_build_dup(q->N, d_fc, fc_stride, q->d_dup, s));
   if (ev0) EDB_HIP_TRY(hipEventRecord(ev0, s));
-  if (fp32)
+  if (src == kPlainFP32)
     EDB_TRY(launch_calq_plain32(q->N, q->d_plain32, q->d_dup, d_fc, fc_stride, q->d_part, q->nslab, s));
+  else if (src == kPlainCompressed)
+    EDB_TRY(launch_calq_plain_otf(q->N, q->tables.rval, q->d_ftab, q->d_sb, q->d_dup, d_fc, fc_stride, q->d_part,
+                                  q->nslab, s));
   else
     EDB_TRY(launch_calq_plain64(q->N, q->d_plain, q->d_dup, d_fc, fc_stride, q->d_part, q->nslab, s));
   if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
@@ -180,7 +184,8 @@ int edb_sync(void) {
 int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out) {
   EDB_REQUIRE(out, "edb_qcalc_create: null output");
   *out = nullptr;
-  EDB_REQUIRE(N >= 3 && N <= kMaxNCalq, "edb_qcalc_create: N must be in [3, 33]");
+  // batched kernels exist up to N = 33; the plain paths (stored or compressed beta) run up to 64
+  EDB_REQUIRE(N >= 3 && N <= kMaxN, "edb_qcalc_create: N must be in [3, 64]");
   EDB_REQUIRE(alpha == 0 || alpha == 1, "edb_qcalc_create: alpha must be 0 or 1");
   EDB_REQUIRE(L > 0. && std::isfinite(L), "edb_qcalc_create: L must be positive");
   edb_qcalc* q = new edb_qcalc();
@@ -207,7 +212,7 @@ int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out) {
 
 int edb_qcalc_close_filter(edb_qcalc* q) {
   EDB_REQUIRE(q, "null handle");
-  EDB_REQUIRE(!q->d_plain && !q->d_plain32 && !q->d_pair, "closeFilter must precede genB");
+  EDB_REQUIRE(!q->d_plain && !q->d_plain32 && !q->d_pair && !q->d_ftab, "closeFilter must precede genB");
   q->filter = 0;
   if (q->tables_ready) {
     beta_tables_free(&q->tables);
@@ -235,6 +240,28 @@ int edb_qcalc_genB_plain32(edb_qcalc* q) {
   EDB_TRY(dmalloc(&q->d_plain32, (size_t)q->ntile * q->g.NQ * 64));
   EDB_TRY(launch_genb_plain32(q->tables, q->d_plain32, q->ntile, q->stream));
   EDB_HIP_TRY(hipStreamSynchronize(q->stream));
+  return EDB_OK;
+}
+
+int edb_qcalc_genB_compressed(edb_qcalc* q) {
+  EDB_REQUIRE(q, "null handle");
+  std::lock_guard<std::mutex> lk(q->mu);
+  if (q->d_ftab) return EDB_OK;
+  EDB_TRY(ensure_tables(q));
+  long R, Rh;
+  compressed_beta_dims(q->N, &R, &Rh);
+  EDB_TRY(dmalloc(&q->d_ftab, (size_t)(R * Rh)));
+  EDB_TRY(dmalloc(&q->d_sb, (size_t)q->g.NQ * 2));
+  EDB_TRY(launch_genb_compressed(q->tables, q->d_ftab, q->d_sb, q->stream));
+  EDB_HIP_TRY(hipStreamSynchronize(q->stream));
+  return EDB_OK;
+}
+
+int edb_qcalc_compressed_bytes(const edb_qcalc* q, long* bytes) {
+  EDB_REQUIRE(q && bytes, "compressed_bytes: null argument");
+  long R, Rh;
+  compressed_beta_dims(q->N, &R, &Rh);
+  *bytes = (R * Rh + 2 * q->g.NQ) * (long)sizeof(double);
   return EDB_OK;
 }
 
@@ -273,14 +300,14 @@ int edb_qcalc_genB_batched(edb_qcalc* q) {
 }
 
 namespace {
-int calq_device_impl(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream, bool fp32) {
+int calq_device_impl(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream, int src) {
   EDB_REQUIRE(q && d_qout && d_fin, "calQ_device: null argument");
   hipStream_t s = stream ? (hipStream_t)stream : q->stream;
   double *fc = nullptr, *qc = nullptr;
   EDB_TRY(dmalloc(&fc, (size_t)q->g.NQ * 2));
   int st = dmalloc(&qc, (size_t)q->g.NQ * 2);
   if (st == kOk) st = launch_pack(q->N, 1, 1, false, fc, d_fin, nullptr, 0., s);
-  if (st == kOk) st = plain_column(q, fc, 1, qc, 1, s, nullptr, nullptr, fp32);
+  if (st == kOk) st = plain_column(q, fc, 1, qc, 1, s, nullptr, nullptr, src);
   if (st == kOk && q->conserve) st = launch_moment_correct(q->d_mom, q->mom_E, 1, 1, qc, s);
   if (st == kOk) st = launch_unpack(q->N, 1, 1, d_qout, qc, s);
   if (st == kOk && hipStreamSynchronize(s) != hipSuccess) {
@@ -292,14 +319,14 @@ int calq_device_impl(edb_qcalc* q, double* d_qout, const double* d_fin, void* st
   return st;
 }
 
-int calq_host_impl(edb_qcalc* q, double* qout, const double* fin, bool fp32) {
+int calq_host_impl(edb_qcalc* q, double* qout, const double* fin, int src) {
   EDB_REQUIRE(q && qout && fin, "calQ: null argument");
   size_t bytes = sizeof(double) * 2 * q->g.N3;
   double *din = nullptr, *dout = nullptr;
   EDB_TRY(dmalloc(&din, (size_t)q->g.N3 * 2));
   int st = dmalloc(&dout, (size_t)q->g.N3 * 2);
   if (st == kOk && hipMemcpy(din, fin, bytes, hipMemcpyHostToDevice) != hipSuccess) st = kHipError;
-  if (st == kOk) st = calq_device_impl(q, dout, din, q->stream, fp32);
+  if (st == kOk) st = calq_device_impl(q, dout, din, q->stream, src);
   if (st == kOk && hipMemcpy(qout, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess) st = kHipError;
   if (st == kHipError && g_err.empty()) set_error("calQ: copy failed");
   dfree(&din);
@@ -309,13 +336,23 @@ int calq_host_impl(edb_qcalc* q, double* qout, const double* fin, bool fp32) {
 }  // namespace
 
 int edb_qcalc_calQ_device(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream) {
-  return calq_device_impl(q, d_qout, d_fin, stream, false);
+  return calq_device_impl(q, d_qout, d_fin, stream, kPlainFP64);
 }
 
-int edb_qcalc_calQ(edb_qcalc* q, double* qout, const double* fin) { return calq_host_impl(q, qout, fin, false); }
+int edb_qcalc_calQ(edb_qcalc* q, double* qout, const double* fin) {
+  return calq_host_impl(q, qout, fin, kPlainFP64);
+}
 
 int edb_qcalc_calQ_plain32(edb_qcalc* q, double* qout, const double* fin) {
-  return calq_host_impl(q, qout, fin, true);
+  return calq_host_impl(q, qout, fin, kPlainFP32);
+}
+
+int edb_qcalc_calQ_compressed(edb_qcalc* q, double* qout, const double* fin) {
+  return calq_host_impl(q, qout, fin, kPlainCompressed);
+}
+
+int edb_qcalc_calQ_compressed_device(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream) {
+  return calq_device_impl(q, d_qout, d_fin, stream, kPlainCompressed);
 }
 
 int edb_qcalc_calQ_batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_qout,
@@ -470,7 +507,7 @@ int edb_qcalc_time_batched(edb_qcalc* q, int batch, int hermitian, int reps, dou
   return batched_steady_ms(q, &T, batch, hermitian, reps, ms);
 }
 
-int edb_qcalc_time_plain(edb_qcalc* q, int reps, double* ms) {
+static int time_plain_src(edb_qcalc* q, int reps, double* ms, int src) {
   EDB_REQUIRE(q && ms && reps >= 1, "time_plain: bad argument");
   const long NQ = q->g.NQ;
   double *fc = nullptr, *dq = nullptr;
@@ -484,9 +521,9 @@ int edb_qcalc_time_plain(edb_qcalc* q, int reps, double* ms) {
       st = kHipError;
   }
   if (st == kOk && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = kHipError;
-  if (st == kOk) st = plain_column(q, fc, 1, dq, 1, q->stream);
+  if (st == kOk) st = plain_column(q, fc, 1, dq, 1, q->stream, nullptr, nullptr, src);
   if (st == kOk && hipEventRecord(e0, q->stream) != hipSuccess) st = kHipError;
-  for (int r = 0; st == kOk && r < reps; ++r) st = plain_column(q, fc, 1, dq, 1, q->stream);
+  for (int r = 0; st == kOk && r < reps; ++r) st = plain_column(q, fc, 1, dq, 1, q->stream, nullptr, nullptr, src);
   if (st == kOk && hipEventRecord(e1, q->stream) != hipSuccess) st = kHipError;
   if (st == kOk && hipEventSynchronize(e1) != hipSuccess) st = kHipError;
   float el = 0.f;
@@ -498,6 +535,12 @@ int edb_qcalc_time_plain(edb_qcalc* q, int reps, double* ms) {
   dfree(&fc);
   dfree(&dq);
   return st;
+}
+
+int edb_qcalc_time_plain(edb_qcalc* q, int reps, double* ms) { return time_plain_src(q, reps, ms, kPlainFP64); }
+
+int edb_qcalc_time_compressed(edb_qcalc* q, int reps, double* ms) {
+  return time_plain_src(q, reps, ms, kPlainCompressed);
 }
 
 int edb_qcalc_tune(edb_qcalc* q, int batched_waves, int plain_slabs) {
@@ -606,6 +649,8 @@ int edb_qcalc_destroy(edb_qcalc* q) {
   if (q->stream) (void)hipStreamSynchronize(q->stream);
   dfree(&q->d_mom);
   dfree(&q->d_plain32);
+  dfree(&q->d_ftab);
+  dfree(&q->d_sb);
   dfree(&q->d_plain);
   dfree(&q->d_dup);
   dfree(&q->d_part);
@@ -761,7 +806,11 @@ struct edb_solver {
 namespace {
 
 bool batched(const edb_solver* s) { return s->p.precision == EDB_PREC_FP32_BATCHED; }
-bool plain32(const edb_solver* s) { return s->p.precision == EDB_PREC_FP32_PLAIN; }
+int plain_source(const edb_solver* s) {
+  return s->p.precision == EDB_PREC_FP32_PLAIN
+             ? kPlainFP32
+             : (s->p.precision == EDB_PREC_FP64_COMPRESSED ? kPlainCompressed : kPlainFP64);
+}
 
 int next_events(edb_solver* s, hipEvent_t* e0, hipEvent_t* e1) {
   *e0 = *e1 = nullptr;
@@ -791,7 +840,7 @@ int qeval(edb_solver* s, const double* X, const double* Qprev, double c0, double
       hipEvent_t e0, e1;
       EDB_TRY(next_events(s, &e0, &e1));
       EDB_TRY(plain_column(s->qc, (const double*)s->in + 2 * i, ld, Qout + 2 * i, ld, s->stream, e0, e1,
-                           plain32(s)));
+                           plain_source(s)));
     }
   }
   if (s->p.conserve) EDB_TRY(launch_moment_correct(s->qc->d_mom, s->qc->mom_E, batch, ld, Qout, s->stream));
@@ -1006,7 +1055,7 @@ int edb_solver_create(const edb_solver_params* p, edb_qcalc* shared, edb_solver*
   *out = nullptr;
   EDB_REQUIRE(p->batch >= 1 && p->batch <= 65536, "solver_create: batch must be in [1, 65536]");
   EDB_REQUIRE(p->precision == EDB_PREC_FP64_PLAIN || p->precision == EDB_PREC_FP32_BATCHED ||
-                  p->precision == EDB_PREC_FP32_PLAIN,
+                  p->precision == EDB_PREC_FP32_PLAIN || p->precision == EDB_PREC_FP64_COMPRESSED,
               "solver_create: unknown precision");
   EDB_REQUIRE(p->scheme >= 0 && p->scheme <= 5, "solver_create: unknown scheme");
   EDB_REQUIRE(p->fix_mode >= 0 && p->fix_mode <= 2, "solver_create: unknown fix mode");
@@ -1033,7 +1082,10 @@ int edb_solver_create(const edb_solver_params* p, edb_qcalc* shared, edb_solver*
   }
   if (st == kOk)
     st = batched(s) ? edb_qcalc_genB_batched(s->qc)
-                    : (plain32(s) ? edb_qcalc_genB_plain32(s->qc) : edb_qcalc_genB(s->qc));
+                    : (plain_source(s) == kPlainFP32
+                           ? edb_qcalc_genB_plain32(s->qc)
+                           : (plain_source(s) == kPlainCompressed ? edb_qcalc_genB_compressed(s->qc)
+                                                                  : edb_qcalc_genB(s->qc)));
   if (st == kOk && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("solver_create: stream creation failed");
     st = kHipError;

@@ -54,6 +54,9 @@ struct edb_qcalc {
   // plain FP64 (and its FP32 copy)
   double* d_plain = nullptr;
   float* d_plain32 = nullptr;
+  // compressed beta: F table over (|l+m|^2, |l-m|^2 / 2) and (sigma, Bmm) per mode
+  double* d_ftab = nullptr;
+  double* d_sb = nullptr;
   int ntile = 0;
   int nslab = 16;
   double* d_dup = nullptr;
@@ -105,7 +108,9 @@ int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, 
 int ensure_plain_scratch(edb_qcalc* q);
 // Plain FP64 Q of one column (centred FP64 input with stride fc_stride).
 // Plain Q of one column with the FP64 beta, or its FP32 copy when fp32.
+// Where the plain kernel takes beta from.
+enum PlainSource { kPlainFP64 = 0, kPlainFP32 = 1, kPlainCompressed = 2 };
 int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, int q_stride,
-                 hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, bool fp32 = false);
+                 hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, int src = kPlainFP64);
 
 }  // namespace edb

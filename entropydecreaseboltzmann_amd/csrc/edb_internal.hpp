@@ -99,6 +99,12 @@ int launch_genb_plain64(const BetaTables& t, double* d_B, int ntile, hipStream_t
 int launch_genb_plain32(const BetaTables& t, float* d_B, int ntile, hipStream_t s);
 int launch_genb_pair32(const BetaTables& t, const int* d_rowinfo, const uint32_t* d_sched,
                        int nrows, long nblocks, float* d_B, hipStream_t s);
+// Compressed beta: the F table (R x Rh doubles) and the per-mode
+// (sigma product, Bmm) pairs (M^3 double2) that beta is formed from.
+void compressed_beta_dims(int N, long* R, long* Rh);
+int launch_genb_compressed(const BetaTables& t, double* d_T, double* d_SB, hipStream_t s);
+int launch_calq_plain_otf(int N, double rval, const double* d_T, const double* d_SB, const double* d_dup,
+                          const double* d_fc, int fc_stride, double* d_part, int nslab, hipStream_t s);
 
 // plain FP64 Q (calq_plain.hip)
 int launch_build_dup(int N, const double* d_fc, int fc_stride, double* d_dup, hipStream_t s);

@@ -143,6 +143,33 @@ __global__ void genb_pair32_kernel(BetaArgs a, const uint32_t* __restrict__ sche
   B[idx] = out;
 }
 
+// Compressed beta.  beta(l, m) depends on (l, m) only through
+// A = |l+m|^2, Bq = |l-m|^2, sigma_l, sigma_m and Bmm[m]; A + Bq = 2|l|^2 +
+// 2|m|^2 is even, so F is tabulated over A in [0, R) and Bq/2 in [0, Rh),
+// R = 12n^2 + 1, Rh = 6n^2 + 1: T[A * Rh + Bq/2] = F_alpha(lambda sqrt(A),
+// lambda sqrt(Bq)), the same device evaluation beta_lk makes, so beta formed
+// from the tables equals the stored beta bit for bit.  Entries whose Bq would
+// exceed R - 1 are never read and hold 0.
+__global__ void genb_ftable_kernel(int alpha, double lambda, long R, long Rh, double* __restrict__ T) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= R * Rh) return;
+  long A = idx / Rh;
+  long Bq = 2 * (idx % Rh) + (A & 1);
+  T[idx] = Bq < R ? funF_lattice(alpha, A, Bq, lambda) : 0.;
+}
+
+// Per centred mode c: (sigma_c1 sigma_c2 sigma_c3, Bmm[c]), the product in
+// beta_lk's order; (1, Bmm[c]) with the filter off.
+__global__ void genb_sigbmm_kernel(BetaArgs a, double2* __restrict__ SB, long NQ) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= NQ) return;
+  int M = a.M;
+  int c1 = (int)(c / ((long)M * M)), c2 = (int)((c / M) % M), c3 = (int)(c % M);
+  double sg = 1.;
+  if (a.filter) sg = a.sig[c1] * a.sig[c2] * a.sig[c3];
+  SB[c] = make_double2(sg, a.bmm[c]);
+}
+
 BetaArgs make_args(const BetaTables& t) {
   Geometry g(t.N);
   BetaArgs a;
@@ -226,6 +253,25 @@ int launch_genb_pair32(const BetaTables& t, const int* d_rowinfo, const uint32_t
   long total = nblocks * g.M * g.M;
   hipLaunchKernelGGL(genb_pair32_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, s,
                      make_args(t), d_sched, nblocks, d_B);
+  EDB_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+void compressed_beta_dims(int N, long* R, long* Rh) {
+  Geometry g(N);
+  *R = 12L * g.n * g.n + 1;
+  *Rh = 6L * g.n * g.n + 1;
+}
+
+int launch_genb_compressed(const BetaTables& t, double* d_T, double* d_SB, hipStream_t s) {
+  Geometry g(t.N);
+  long R, Rh;
+  compressed_beta_dims(t.N, &R, &Rh);
+  hipLaunchKernelGGL(genb_ftable_kernel, dim3(blocks_for(R * Rh, 256)), dim3(256), 0, s, t.alpha, t.lambda, R,
+                     Rh, d_T);
+  EDB_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(genb_sigbmm_kernel, dim3(blocks_for(g.NQ, 256)), dim3(256), 0, s, make_args(t),
+                     (double2*)d_SB, g.NQ);
   EDB_HIP_TRY(hipGetLastError());
   return kOk;
 }

@@ -47,6 +47,8 @@ extern "C" {
 #define EDB_PREC_FP64_PLAIN 0    /* one instance at a time, FP64 beta streamed */
 #define EDB_PREC_FP32_BATCHED 1  /* all instances in one launch, FP32 beta */
 #define EDB_PREC_FP32_PLAIN 2    /* one instance at a time, FP32 beta streamed, FP64 products */
+#define EDB_PREC_FP64_COMPRESSED 3 /* one instance at a time, FP64 beta formed on the fly from the
+                                      compressed tables (edb_qcalc_genB_compressed) */
 
 #define EDB_SCHEME_FEULER 0  /* BKW3D::FEuler, BKW.h:191-251 */
 #define EDB_SCHEME_RK2 1     /* BKW3D::RK2,    BKW.h:314-384 */
@@ -75,7 +77,9 @@ typedef struct edb_qcalc edb_qcalc;
 /* QCalculator(int N, int alpha), QCalculator.h:39 + initialize() :43-52.
  * L is the velocity half-width that QCalculator.h:176 hard-codes as
  * 6.62132.  The Jackson filter starts on, as IS_FILTER = 1 at
- * QCalculator.h:34.  alpha: 0 Maxwell molecules, 1 hard spheres. */
+ * QCalculator.h:34.  alpha: 0 Maxwell molecules, 1 hard spheres.  N in [3, 64]; the
+ * batched kernels (genB_batched) and the solver exist up to N = 33, the plain
+ * paths up to 64. */
 int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out);
 
 /* closeFilter(), QCalculator.h:54.  Must precede genB. */
@@ -92,12 +96,28 @@ int edb_qcalc_genB_batched(edb_qcalc* q);
  * single-instance FP32-beta path: half the bytes of the plain HBM stream,
  * products and sums in FP64.  No reference counterpart (its B is FP64). */
 int edb_qcalc_genB_plain32(edb_qcalc* q);
+/* Compressed beta for the plain FP64 path, in place of genB's M^6 array:
+ * beta(l,m) = F(lambda|l+m|, lambda|l-m|) sigma_l sigma_m rval - Bmm[m]
+ * (QCalculator.h:199-228) depends on (l, m) only through |l+m|^2, |l-m|^2,
+ * the filter factors and Bmm[m], so F is tabulated over those two integers
+ * ((12n^2+1) x (6n^2+1) doubles: 8.5 MB at N=24, 0.53 GB at N=64) beside
+ * M^3 (sigma, Bmm) pairs, and the kernel forms each beta on the fly.  Equal
+ * to genB's beta bit for bit.  No reference counterpart (QCalculator.h
+ * stores the N^6 array). */
+int edb_qcalc_genB_compressed(edb_qcalc* q);
+/* Device bytes of the compressed tables for q's N. */
+int edb_qcalc_compressed_bytes(const edb_qcalc* q, long* bytes);
 
 /* calQ(Qout, fin), QCalculator.h:59, 231-257: one Q-evaluation of an
  * N^3 complex spectrum in FFTW order, FP64 plain kernel.  Host arrays. */
 int edb_qcalc_calQ(edb_qcalc* q, double* qout, const double* fin);
 /* calQ with the FP32 copy of beta (edb_qcalc_genB_plain32). */
 int edb_qcalc_calQ_plain32(edb_qcalc* q, double* qout, const double* fin);
+/* calQ with beta formed from the compressed tables (edb_qcalc_genB_compressed):
+ * the result of edb_qcalc_calQ, bit for bit, with no M^6 array; host and
+ * device arrays as edb_qcalc_calQ / edb_qcalc_calQ_device. */
+int edb_qcalc_calQ_compressed(edb_qcalc* q, double* qout, const double* fin);
+int edb_qcalc_calQ_compressed_device(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream);
 
 /* Same on device arrays (FFTW-order N^3 complex). */
 int edb_qcalc_calQ_device(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream);
@@ -121,11 +141,15 @@ int edb_qcalc_copy_B(edb_qcalc* q, double* Bout);
 int edb_qcalc_info(const edb_qcalc* q, int* N, int* M, long* bytes_plain64, long* bytes_pair32);
 
 /* Time the batched kernel alone: reps launches on the handle's stream,
- * bracketed by HIP events; returns the mean milliseconds per launch. */
+ * bracketed by HIP events, after ~40 ms of warm-up launches (the device's
+ * clock settles over the first launches after idle); returns the mean
+ * milliseconds per launch. */
 int edb_qcalc_time_batched(edb_qcalc* q, int batch, int hermitian, int reps, double* ms);
 
 /* Same for the plain FP64 kernel (one instance). */
 int edb_qcalc_time_plain(edb_qcalc* q, int reps, double* ms);
+/* Same for the plain kernel with beta formed from the compressed tables. */
+int edb_qcalc_time_compressed(edb_qcalc* q, int reps, double* ms);
 
 /* Launch-shape knobs: persistent waves of the batched kernel and l-slabs of
  * the plain kernel (0 = built-in default).  No reference counterpart. */

@@ -54,6 +54,7 @@ def lib():
         o.edbo_genB.argtypes = [I, I, D, I, _dp]
         o.edbo_calQ.argtypes = [I, _dp, _dp, _dp, I]
         o.edbo_calQ_direct.argtypes = [I, I, D, I, _dp, _dp]
+        o.edbo_calQ_modes.argtypes = [I, I, D, I, _dp, I, ctypes.POINTER(ctypes.c_long), I, _dp]
         o.edbo_dft3.argtypes = [I, I, _dp, _dp]
         o.edbo_fft.argtypes = [I, _dp, _dp]
         o.edbo_ifft.argtypes = [I, _dp, _dp]
@@ -108,6 +109,18 @@ def calQ_direct(N, fin, alpha=0, L=6.62132, filter=True):
     lib().edbo_calQ_direct(N, alpha, L, int(bool(filter)), _p(f.view(np.float64)),
                            _p(q.view(np.float64)))
     return q
+
+
+def calQ_modes(N, fin, kcs, alpha=0, L=6.62132, filter=True, absval=False):
+    """Q (or, with absval, the sum of |terms|) at centred mode indices kcs,
+    beta formed per term: no M^6 array."""
+    f = np.ascontiguousarray(fin, dtype=np.complex128).reshape(N, N, N)
+    k = np.ascontiguousarray(kcs, dtype=np.int64)
+    out = np.empty(k.size, dtype=np.complex128)
+    lib().edbo_calQ_modes(N, alpha, L, int(bool(filter)), _p(f.view(np.float64)), int(k.size),
+                          k.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), int(bool(absval)),
+                          _p(out.view(np.float64)))
+    return out
 
 
 def fft(re):

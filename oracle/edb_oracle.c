@@ -19,6 +19,9 @@
  *   edbo_calQ, edbo_calQ_direct
  *       golden Q from QCalculator::calQ (odd N); brute-force (l,m)-order sum
  *       (even N too); Q_0 == 0; Hermitian symmetry; Q(Maxwellian) ~ 0.
+ *   edbo_calQ_modes
+ *       equal bit for bit to edbo_calQ with edbo_genB's B on the selected
+ *       modes (odd and even N), which the checks above pin.
  *   edbo_dft3, edbo_fft, edbo_ifft      numpy.fft.fftn / ifftn.
  *   edbo_set_average, edbo_entropy_cal, edbo_fixsol
  *       golden values from the compiled EntropyFix.h (testEntropyFix inputs
@@ -289,6 +292,58 @@ void edbo_calQ_direct(int N, int alpha, double L, int filter, const double* fin,
               qout[2 * jk + 1] += (ar * ci + ai * cr) * b;
             }
       }
+}
+
+/* Q at selected modes without the M^6 array: for each centred mode index
+ * kc[i], Q_k = sum_l (f_l * f_[k-l]) * beta(l, [k-l]) in edbo_calQ's l order
+ * and arithmetic, with each beta formed by edbo_beta (the per-entry formula
+ * of QCalculator.h:199-228).  absval != 0 accumulates |beta| |f_l| |f_[k-l]|
+ * into the real part instead (the scale of the summed terms).  fin: N^3
+ * complex in FFTW order; out: nk complex.  Cost M^3 beta evaluations per
+ * mode, so it serves grids whose M^6 array would not fit. */
+void edbo_calQ_modes(int N, int alpha, double L, int filter, const double* fin, int nk, const long* kc,
+                     int absval, double* out) {
+  int n = (N - 1) / 2, M = edbo_nmodes(N);
+  long NQ = (long)M * M * M;
+  double* fc = (double*)malloc(sizeof(double) * 2 * NQ);
+  for (long c = 0; c < NQ; ++c) {
+    int c1 = (int)(c / ((long)M * M)), c2 = (int)((c / M) % M), c3 = (int)(c % M);
+    long j = ((long)fftw_index(N, c1 - n) * N + fftw_index(N, c2 - n)) * N + fftw_index(N, c3 - n);
+    fc[2 * c] = fin[2 * j];
+    fc[2 * c + 1] = fin[2 * j + 1];
+  }
+  int* wt = (int*)malloc(sizeof(int) * (2 * M - 1));
+  for (int d = -(M - 1); d <= M - 1; ++d) {
+    int v, w = edbo_wrap(N, d, &v);
+    wt[d + M - 1] = v ? w + n : -1;
+  }
+#pragma omp parallel for schedule(dynamic)
+  for (int i = 0; i < nk; ++i) {
+    long k = kc[i];
+    int c1 = (int)(k / ((long)M * M)), c2 = (int)((k / M) % M), c3 = (int)(k % M);
+    double qr = 0., qi = 0.;
+    for (long lc = 0; lc < NQ; ++lc) {
+      int l1 = (int)(lc / ((long)M * M)), l2 = (int)((lc / M) % M), l3 = (int)(lc % M);
+      int m1 = wt[c1 - l1 + M - 1], m2 = wt[c2 - l2 + M - 1], m3 = wt[c3 - l3 + M - 1];
+      if (m1 < 0 || m2 < 0 || m3 < 0) continue;
+      long m = ((long)m1 * M + m2) * M + m3;
+      double ar = fc[2 * lc], ai = fc[2 * lc + 1];
+      double cr = fc[2 * m], ci = fc[2 * m + 1];
+      double b = edbo_beta(N, alpha, L, filter, l1 - n, l2 - n, l3 - n, m1 - n, m2 - n, m3 - n);
+      if (absval) {
+        qr += fabs(b) * sqrt(ar * ar + ai * ai) * sqrt(cr * cr + ci * ci);
+      } else {
+        double pr = ar * cr - ai * ci;
+        double pi = ar * ci + ai * cr;
+        qr += pr * b;
+        qi += pi * b;
+      }
+    }
+    out[2 * i] = qr;
+    out[2 * i + 1] = qi;
+  }
+  free(wt);
+  free(fc);
 }
 
 /* ------------------------------------------------------------------ DFT -- */

@@ -50,6 +50,8 @@ void   edbo_calQ(int N, const double* B, const double* fin, double* qout,
                  int nthreads);
 void   edbo_calQ_direct(int N, int alpha, double L, int filter,
                         const double* fin, double* qout);
+void   edbo_calQ_modes(int N, int alpha, double L, int filter, const double* fin, int nk,
+                       const long* kc, int absval, double* out);
 
 void   edbo_dft3(int N, int sign, const double* in, double* out);
 void   edbo_fft(int N, const double* re, double* fre);

@@ -1,0 +1,90 @@
+"""Compressed beta (edb_qcalc_genB_compressed / calQ_compressed): beta formed
+on the fly from F tabulated over (|l+m|^2, |l-m|^2) and the per-mode (sigma,
+Bmm) pairs, in place of the M^6 array of QCalculator::genB
+(QCalculator.h:158-229).
+
+* where both fit, the result equals the stored-beta FP64 kernel bit for bit;
+* at N = 48 and 64, where the FP64 M^6 array needs 86 GB and 500 GB, selected
+  modes are checked against the oracle's per-mode sum (beta from the
+  reference's per-entry formula, no M^6 array) to 1e-12 of the summed |terms|,
+  Q_0 is exactly 0 and Q is Hermitian for real data."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import maxwellian_mixture, rand_spectrum
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("N,alpha,fil", [(9, 0, 0), (9, 1, 1), (10, 0, 1), (16, 1, 0), (24, 0, 0),
+                                         (25, 1, 1)])
+def test_compressed_equals_stored_bitwise(edb, N, alpha, fil):
+    q = edb.QCalculator(N, alpha, 8.0)
+    if not fil:
+        q.closeFilter()
+    q.genB()
+    q.genB_compressed()
+    for seed in (1, 2):
+        f = rand_spectrum(N, 100 * N + seed)
+        np.testing.assert_array_equal(q.calQ_compressed(f), q.calQ(f))
+    if N >= 16:
+        assert q.compressed_bytes() < q.info()["bytes_plain64"] / 50
+
+
+def _centred_to_fftw(N, kcs):
+    n, M = (N - 1) // 2, oracle.nmodes(N)
+    c1, c2, c3 = kcs // (M * M), (kcs // M) % M, kcs % M
+    return (((c1 - n) % N) * N + (c2 - n) % N) * N + (c3 - n) % N
+
+
+def _negate(N, kcs):
+    M = oracle.nmodes(N)
+    c1, c2, c3 = kcs // (M * M), (kcs // M) % M, kcs % M
+    return ((M - 1 - c1) * M + (M - 1 - c2)) * M + (M - 1 - c3)
+
+
+@pytest.mark.parametrize("N,alpha", [(48, 0), (64, 0), (64, 1)])
+def test_compressed_large_grid_matches_oracle_modes(edb, N, alpha):
+    L = 8.0
+    M = oracle.nmodes(N)
+    q = edb.QCalculator(N, alpha, L)
+    q.closeFilter()
+    assert q.info()["bytes_plain64"] > 80e9  # the stored FP64 array would not be practical
+    q.genB_compressed()
+    f = np.fft.fftn(maxwellian_mixture(N, L, 7 + N))
+    Q = q.calQ_compressed(f).ravel()
+    rng = np.random.default_rng(N + alpha)
+    kcs = np.concatenate([[M ** 3 // 2 + 1], rng.choice(M ** 3, 5, replace=False)])
+    kcs = np.concatenate([kcs, _negate(N, kcs)])
+    ref = oracle.calQ_modes(N, f, kcs, alpha, L, False)
+    scale = oracle.calQ_modes(N, f, kcs, alpha, L, False, absval=True).real
+    got = Q[_centred_to_fftw(N, kcs)]
+    assert np.all(np.abs(got - ref) <= 1e-12 * scale), np.abs(got - ref) / scale
+    # k = 0: every term has m = -l, where beta is exactly 0
+    assert Q[0] == 0
+    # Hermitian symmetry of Q for real data, over the whole grid
+    Qg = Q.reshape(N, N, N)
+    Qm = np.conj(np.roll(np.flip(Qg), 1, axis=(0, 1, 2)))  # conj Q at -k (mod N)
+    assert np.abs(Qg - Qm).max() <= 1e-12 * scale.max()
+
+
+def test_solver_compressed_equals_plain(edb):
+    N, L, dt = 9, 8.0, 0.05
+    F0 = np.stack([maxwellian_mixture(N, L, 31 + i) for i in range(2)])
+    out = {}
+    for prec in (edb.PREC_FP64_PLAIN, edb.PREC_FP64_COMPRESSED):
+        s = edb.Solver(N, 0, L, False, 2, prec, "RK4", dt, 0.0, edb.FIX_APPLIED, history=8)
+        s.set_initial_real(F0)
+        s.step(6)
+        out[prec] = s.get_spectral()
+    np.testing.assert_array_equal(out[edb.PREC_FP64_COMPRESSED], out[edb.PREC_FP64_PLAIN])
+
+
+def test_time_compressed(edb):
+    q = edb.QCalculator(24, 0, 8.0)
+    q.closeFilter()
+    q.genB_compressed()
+    assert q.time_compressed(reps=2) > 0
+    with pytest.raises(edb.EdbError):
+        q.calQ(rand_spectrum(24, 1))  # the stored array was never built

@@ -92,6 +92,29 @@ def test_calQ_matches_direct_sum(N, alpha, fil):
     np.testing.assert_allclose(q, qd, rtol=0, atol=1e-13 * scale)
 
 
+def _centred_to_fftw(N, kcs):
+    n, M = (N - 1) // 2, oracle.nmodes(N)
+    c1, c2, c3 = kcs // (M * M), (kcs // M) % M, kcs % M
+    return (((c1 - n) % N) * N + (c2 - n) % N) * N + (c3 - n) % N
+
+
+@pytest.mark.parametrize("N,nk", [(5, 0), (8, 0), (9, 64), (10, 64)])
+@pytest.mark.parametrize("alpha,fil", [(0, 0), (1, 1), (0, 1)])
+def test_calQ_modes_matches_calQ_bitwise(N, nk, alpha, fil):
+    """Q at selected modes with beta formed per term equals calQ with the
+    stored B bit for bit; the |term| sum bounds |Q|."""
+    M = oracle.nmodes(N)
+    f = rand_spectrum(N, 7 * N + alpha + 3 * fil)
+    B = oracle.genB(N, alpha, 8.0, fil)
+    q = oracle.calQ(N, B, f).ravel()
+    rng = np.random.default_rng(N)
+    kcs = np.arange(M ** 3) if nk == 0 else rng.choice(M ** 3, nk, replace=False)
+    qm = oracle.calQ_modes(N, f, kcs, alpha, 8.0, fil)
+    np.testing.assert_array_equal(qm, q[_centred_to_fftw(N, kcs)])
+    s = oracle.calQ_modes(N, f, kcs, alpha, 8.0, fil, absval=True)
+    assert np.all(s.imag == 0) and np.all(np.abs(qm) <= s.real * (1 + 1e-12))
+
+
 def test_calQ_planted_mistake_is_caught():
     """A transposed B (beta(m,l) instead of beta(l,m)) fails the direct sum."""
     N = 5
