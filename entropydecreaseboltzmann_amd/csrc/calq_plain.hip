@@ -102,78 +102,82 @@ __device__ __forceinline__ int wrap_small(int N, int n, int d, bool* ok) {
 
 // The plain evaluation with beta formed on the fly from the compressed tables
 // (genb.hip): beta(l, m) = T[|l+m|^2 Rh + |l-m|^2 / 2] sigma_l sigma_m rval -
-// Bmm[m], in the stored kernel's order, so the result is the stored FP64
-// kernel's bit for bit.  Nothing of size M^6 exists: the reads per term are
-// the F entry (L2/MALL-resident table), the (sigma, Bmm) pair of m and the
-// f_[k-l] gather.  The (l1, l2) parts of the index arithmetic are kept per
-// l-row; each term does the third dimension only.
-template <int UNROLL>
+// Bmm[m], in genb's order, so every beta equals the stored one bit for bit.
+// Nothing of size M^6 exists.
+//
+// Lanes run over 64 consecutive l and a wave owns K consecutive k3 of one
+// k-row (k1, k2), wave-uniform: |l+m|^2 = |k - N w|^2 (w the wrap of k - l)
+// depends on k alone wherever no wrap occurs, so the lanes of a wave read one
+// row of T and the row stays in L1 across the wave's sweep of l; the f_[k-l]
+// and (sigma, Bmm)[m] reads are contiguous across lanes (m = k - l runs
+// backwards with l).  Each lane accumulates its l terms for the K modes in
+// l order; the 64 lanes are then summed with a fixed butterfly, and slabs of
+// l are summed afterwards in a fixed order, so the result is deterministic
+// (it differs from the stored kernel only in summation order).
+template <int K>
 __global__ __launch_bounds__(64) void calq_plain_otf_kernel(int N, int M, long NQ, double rval,
-                                                            const double* __restrict__ T, long Rh,
+                                                            const double* __restrict__ T, int Rh,
                                                             const double2* __restrict__ SB,
                                                             const double2* __restrict__ dup,
                                                             const double2* __restrict__ fc, int stride,
                                                             double2* __restrict__ part, int nslab) {
   const int n = (N - 1) / 2, W = 2 * M - 1;
   const int lane = threadIdx.x;
-  const int tile = blockIdx.x;
-  const int slab = blockIdx.y;
-  long kc = (long)tile * 64 + lane;
-  long kcc = kc < NQ ? kc : NQ - 1;  // padded lanes compute a valid term that is never written
-  int k1 = (int)(kcc / ((long)M * M)), k2 = (int)((kcc / M) % M), k3 = (int)(kcc % M);
-  long dbase = ((long)(k1 + M - 1) * W + (k2 + M - 1)) * W + (k3 + M - 1);
-  long l0 = NQ * slab / nslab, l1e = NQ * (slab + 1) / nslab;
-  int a1 = (int)(l0 / ((long)M * M)), a2 = (int)((l0 / M) % M), a3 = (int)(l0 % M);
-  // per l-row (a1, a2): validity and the (1, 2) parts of |l+m|^2, |l-m|^2, m
-  bool rowok;
-  long A12, B12, mc12;
-  auto row = [&]() {
+  const int nk3 = (M + K - 1) / K;
+  const int kb = blockIdx.x, slab = blockIdx.y;
+  const int k1 = kb / (M * nk3), k2 = (kb / nk3) % M, k3b = (kb % nk3) * K;
+  const int dk12 = ((k1 + M - 1) * W + (k2 + M - 1)) * W + (M - 1);
+  const int l0 = (int)(NQ * slab / nslab), l1e = (int)(NQ * (slab + 1) / nslab);
+  double ar[K], ai[K];
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    ar[u] = 0.;
+    ai[u] = 0.;
+  }
+  for (int lb = l0; lb < l1e; lb += 64) {
+    const bool active = lb + lane < l1e;
+    const int l = active ? lb + lane : l1e - 1;  // inactive lanes read valid addresses, add 0
+    const int row = l / M, a3 = l - row * M;
+    const int a1 = row / M, a2 = row - a1 * M;
+    const double2 fl = fc[(long)l * stride];
+    const double sl = SB[l].x;
     bool v1, v2;
     const int m1 = wrap_small(N, n, k1 - a1, &v1), m2 = wrap_small(N, n, k2 - a2, &v2);
-    const int l1 = a1 - n, l2 = a2 - n;
-    rowok = v1 && v2;
-    A12 = (long)(l1 + m1) * (l1 + m1) + (long)(l2 + m2) * (l2 + m2);
-    B12 = (long)(l1 - m1) * (l1 - m1) + (long)(l2 - m2) * (l2 - m2);
-    mc12 = ((long)(m1 + n) * M + (m2 + n)) * M + n;
-  };
-  auto beta = [&](long l) -> double {
+    const int l1 = a1 - n, l2 = a2 - n, l3 = a3 - n;
+    const bool ok12 = active && v1 && v2;
+    const int A12 = (l1 + m1) * (l1 + m1) + (l2 + m2) * (l2 + m2);
+    const int B12 = (l1 - m1) * (l1 - m1) + (l2 - m2) * (l2 - m2);
+    const int mc12 = ((m1 + n) * M + (m2 + n)) * M + n;
+    const int dl = dk12 - (a1 * W + a2) * W - a3;  // f_[k-l] sits at dl + k3
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
 #pragma clang fp contract(off)
-    bool v3;
-    const int m3 = wrap_small(N, n, k3 - a3, &v3);
-    const int l3 = a3 - n;
-    const long A = A12 + (long)(l3 + m3) * (l3 + m3);
-    const long Bq = B12 + (long)(l3 - m3) * (l3 - m3);
-    const double F = T[A * Rh + (Bq >> 1)];
-    const double2 sb = SB[mc12 + m3];
-    const double v = F * SB[l].x * sb.x * rval - sb.y;
-    return (rowok && v3 && A != 0) ? v : 0.;
-  };
-  row();
-  double qr = 0., qi = 0.;
-  long l = l0;
-  for (; l + UNROLL <= l1e; l += UNROLL) {
-    double b[UNROLL];
-    double2 c[UNROLL], a[UNROLL];
-#pragma unroll
-    for (int u = 0; u < UNROLL; ++u) {
-      long lin = ((long)a1 * W + a2) * W + a3;
-      c[u] = dup[dbase - lin];
-      a[u] = fc[(l + u) * stride];
-      b[u] = beta(l + u);
-      if (++a3 == M) { a3 = 0; if (++a2 == M) { a2 = 0; ++a1; } row(); }
+      // past the row's last k3 the mode is clamped for addressing and adds 0
+      const bool kin = k3b + u < M;
+      const int k3 = kin ? k3b + u : M - 1;
+      bool v3;
+      const int m3 = wrap_small(N, n, k3 - a3, &v3);
+      const int A = A12 + (l3 + m3) * (l3 + m3);
+      const int Bq = B12 + (l3 - m3) * (l3 - m3);
+      const double F = T[A * Rh + (Bq >> 1)];
+      const double2 sb = SB[mc12 + m3];
+      const double2 c = dup[dl + k3];
+      const double v = F * sl * sb.x * rval - sb.y;
+      const double b = (kin && ok12 && v3 && A != 0) ? v : 0.;
+      accum_term(fl, c, b, ar[u], ai[u]);
     }
+  }
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) accum_term(a[u], c[u], b[u], qr, qi);
+  for (int u = 0; u < K; ++u) {
+    double r = ar[u], i = ai[u];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      r += __shfl_xor(r, o);
+      i += __shfl_xor(i, o);
+    }
+    const int k3 = k3b + u;
+    if (lane == 0 && k3 < M) part[(long)slab * NQ + ((long)k1 * M + k2) * M + k3] = make_double2(r, i);
   }
-  for (; l < l1e; ++l) {
-    long lin = ((long)a1 * W + a2) * W + a3;
-    double2 c = dup[dbase - lin];
-    double2 a = fc[l * stride];
-    double b = beta(l);
-    accum_term(a, c, b, qr, qi);
-    if (++a3 == M) { a3 = 0; if (++a2 == M) { a2 = 0; ++a1; } row(); }
-  }
-  if (kc < NQ) part[(long)slab * NQ + kc] = make_double2(qr, qi);
 }
 
 __global__ void reduce_plain_kernel(long NQ, const double2* __restrict__ part, int nslab,
@@ -230,9 +234,9 @@ int launch_calq_plain_otf(int N, double rval, const double* d_T, const double* d
   EDB_REQUIRE(nslab >= 1 && nslab <= 4096 && nslab <= g.NQ, "calq_plain_otf: bad slab count");
   long R, Rh;
   compressed_beta_dims(N, &R, &Rh);
-  int ntile = (int)((g.NQ + 63) / 64);
-  hipLaunchKernelGGL((calq_plain_otf_kernel<8>), dim3(ntile, nslab), dim3(64), 0, s, N, g.M, g.NQ, rval, d_T, Rh,
-                     (const double2*)d_SB, (const double2*)d_dup, (const double2*)d_fc, fc_stride,
+  const int nkb = g.M * g.M * ((g.M + kOtfK - 1) / kOtfK);
+  hipLaunchKernelGGL((calq_plain_otf_kernel<kOtfK>), dim3(nkb, nslab), dim3(64), 0, s, N, g.M, g.NQ, rval, d_T,
+                     (int)Rh, (const double2*)d_SB, (const double2*)d_dup, (const double2*)d_fc, fc_stride,
                      (double2*)d_part, nslab);
   EDB_HIP_TRY(hipGetLastError());
   return kOk;

@@ -124,11 +124,21 @@ int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, 
                                   count, s, ev0, ev1);
 }
 
+// l-slabs of the on-the-fly plain kernel: about eight waves per SIMD, each
+// slab at least 1024 l.
+int otf_slabs(const edb_qcalc* q) {
+  const long nkb = (long)q->g.M * q->g.M * ((q->g.M + kOtfK - 1) / kOtfK);
+  long ns = (cu_count() * 32L + nkb - 1) / nkb;
+  ns = std::min(ns, std::max(1L, q->g.NQ / 1024));
+  return (int)std::max(1L, std::min(ns, 4096L));
+}
+
 int ensure_plain_scratch(edb_qcalc* q) {
   if (q->d_dup) return kOk;
   long W = 2 * q->g.M - 1;
   EDB_TRY(dmalloc(&q->d_dup, (size_t)(W * W * W * 2)));
-  EDB_TRY(dmalloc(&q->d_part, (size_t)(q->nslab * q->g.NQ * 2)));
+  const long ns = std::max<long>(q->nslab, otf_slabs(q));
+  EDB_TRY(dmalloc(&q->d_part, (size_t)(ns * q->g.NQ * 2)));
   return kOk;
 }
 
@@ -146,11 +156,12 @@ int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, i
     EDB_TRY(launch_calq_plain32(q->N, q->d_plain32, q->d_dup, d_fc, fc_stride, q->d_part, q->nslab, s));
   else if (src == kPlainCompressed)
     EDB_TRY(launch_calq_plain_otf(q->N, q->tables.rval, q->d_ftab, q->d_sb, q->d_dup, d_fc, fc_stride, q->d_part,
-                                  q->nslab, s));
+                                  otf_slabs(q), s));
   else
     EDB_TRY(launch_calq_plain64(q->N, q->d_plain, q->d_dup, d_fc, fc_stride, q->d_part, q->nslab, s));
   if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
-  EDB_TRY(launch_reduce_plain(q->N, q->d_part, q->nslab, d_q, q_stride, s));
+  EDB_TRY(launch_reduce_plain(q->N, q->d_part, src == kPlainCompressed ? otf_slabs(q) : q->nslab, d_q, q_stride,
+                                s));
   return kOk;
 }
 

@@ -102,6 +102,9 @@ int launch_genb_pair32(const BetaTables& t, const int* d_rowinfo, const uint32_t
 // Compressed beta: the F table (R x Rh doubles) and the per-mode
 // (sigma product, Bmm) pairs (M^3 double2) that beta is formed from.
 void compressed_beta_dims(int N, long* R, long* Rh);
+// k3 modes per wave of the on-the-fly plain kernel; its grid is
+// M * M * ceil(M / kOtfK) waves per l-slab.
+constexpr int kOtfK = 8;
 int launch_genb_compressed(const BetaTables& t, double* d_T, double* d_SB, hipStream_t s);
 int launch_calq_plain_otf(int N, double rval, const double* d_T, const double* d_SB, const double* d_dup,
                           const double* d_fc, int fc_stride, double* d_part, int nslab, hipStream_t s);

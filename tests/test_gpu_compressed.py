@@ -3,7 +3,8 @@ on the fly from F tabulated over (|l+m|^2, |l-m|^2) and the per-mode (sigma,
 Bmm) pairs, in place of the M^6 array of QCalculator::genB
 (QCalculator.h:158-229).
 
-* where both fit, the result equals the stored-beta FP64 kernel bit for bit;
+* where both fit, the result equals the stored-beta FP64 kernel to FP64
+  rounding (every beta is the same double; the sums run in another order);
 * at N = 48 and 64, where the FP64 M^6 array needs 86 GB and 500 GB, selected
   modes are checked against the oracle's per-mode sum (beta from the
   reference's per-entry formula, no M^6 array) to 1e-12 of the summed |terms|,
@@ -19,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("N,alpha,fil", [(9, 0, 0), (9, 1, 1), (10, 0, 1), (16, 1, 0), (24, 0, 0),
                                          (25, 1, 1)])
-def test_compressed_equals_stored_bitwise(edb, N, alpha, fil):
+def test_compressed_matches_stored(edb, N, alpha, fil):
     q = edb.QCalculator(N, alpha, 8.0)
     if not fil:
         q.closeFilter()
@@ -27,7 +28,11 @@ def test_compressed_equals_stored_bitwise(edb, N, alpha, fil):
     q.genB_compressed()
     for seed in (1, 2):
         f = rand_spectrum(N, 100 * N + seed)
-        np.testing.assert_array_equal(q.calQ_compressed(f), q.calQ(f))
+        qs = q.calQ(f)
+        qc = q.calQ_compressed(f)
+        assert np.abs(qc - qs).max() <= 1e-12 * np.abs(qs).max()
+        # modes without a term (the Nyquist planes of even N, k = 0) are exactly 0 in both
+        np.testing.assert_array_equal(qc == 0, qs == 0)
     if N >= 16:
         assert q.compressed_bytes() < q.info()["bytes_plain64"] / 50
 
@@ -69,7 +74,7 @@ def test_compressed_large_grid_matches_oracle_modes(edb, N, alpha):
     assert np.abs(Qg - Qm).max() <= 1e-12 * scale.max()
 
 
-def test_solver_compressed_equals_plain(edb):
+def test_solver_compressed_matches_plain(edb):
     N, L, dt = 9, 8.0, 0.05
     F0 = np.stack([maxwellian_mixture(N, L, 31 + i) for i in range(2)])
     out = {}
@@ -78,7 +83,8 @@ def test_solver_compressed_equals_plain(edb):
         s.set_initial_real(F0)
         s.step(6)
         out[prec] = s.get_spectral()
-    np.testing.assert_array_equal(out[edb.PREC_FP64_COMPRESSED], out[edb.PREC_FP64_PLAIN])
+    a, b = out[edb.PREC_FP64_COMPRESSED], out[edb.PREC_FP64_PLAIN]
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
 
 
 def test_time_compressed(edb):
