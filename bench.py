@@ -479,6 +479,46 @@ def experiment_check256(edb, N=24, batch=256, steps=200, sample=8):
             "max_linf_rel": max(r["linf_rel"] for r in res), "instances": res}
 
 
+def experiment_compressed(edb, sizes=(24, 33, 48, 64), nmodes=6):
+    """Plain FP64 Q-evaluation with beta formed from the compressed tables at
+    grids up to N = 64, beside the stored-beta kernel where its M^6 array is
+    practical; at each size a few modes are checked against the oracle's
+    per-mode sum (beta from the reference's per-entry formula)."""
+    import oracle
+    prog = lambda m: print(m, file=sys.stderr, flush=True)
+    L = 8.0
+    res = []
+    for N in sizes:
+        M = edb.nmodes(N)
+        q = edb.QCalculator(N, 0, L)
+        q.closeFilter()
+        t0 = time.perf_counter()
+        q.genB_compressed()
+        tgen = time.perf_counter() - t0
+        ms = min(q.time_compressed(reps=3 if N <= 33 else 1) for _ in range(2))
+        row = {"N": N, "M": M, "genB_compressed_ms": round(tgen * 1e3, 2), "ms_per_qeval": round(ms, 3),
+               "terms_per_s": round(M ** 6 / (ms * 1e-3), 0), "table_bytes": q.compressed_bytes(),
+               "stored_beta_bytes": q.info()["bytes_plain64"]}
+        f = np.fft.fftn(mixture(N, L, 77 + N))
+        Q = q.calQ_compressed(f).ravel()
+        rng = np.random.default_rng(N)
+        kcs = rng.choice(M ** 3, nmodes, replace=False)
+        n = (N - 1) // 2
+        c1, c2, c3 = kcs // (M * M), (kcs // M) % M, kcs % M
+        idx = (((c1 - n) % N) * N + (c2 - n) % N) * N + (c3 - n) % N
+        ref = oracle.calQ_modes(N, f, kcs, 0, L, False)
+        scale = oracle.calQ_modes(N, f, kcs, 0, L, False, absval=True).real
+        row["max_err_over_term_sum"] = float((np.abs(Q[idx] - ref) / scale).max())
+        if row["stored_beta_bytes"] < 20e9:
+            q.genB()
+            row["stored_ms_per_qeval"] = round(min(q.time_plain(reps=3) for _ in range(2)), 3)
+        q.close()
+        prog(f"compressed: N={N} done: {ms:.3f} ms per Q-eval")
+        res.append(row)
+    return {"experiment": "plain FP64 Q with compressed beta, N up to 64", "alpha": 0, "L": L,
+            "filter": False, "sizes": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -498,7 +538,7 @@ def main():
     ap.add_argument("--max-pending", type=int, default=0,
                     help="admission limit of the server (--serve): requests beyond it are rejected")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--experiment", choices=["bkw16", "bkw24", "n32", "check256"],
+    ap.add_argument("--experiment", choices=["bkw16", "bkw24", "n32", "check256", "compressed"],
                     help="run one of the reference configurations with its error report")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="no solver: one 256 MiB device copy (the reference transfer for PMC "
@@ -524,6 +564,8 @@ def main():
             res = experiment_bkw(edb, 24)
         elif args.experiment == "n32":
             res = experiment_n32(edb)
+        elif args.experiment == "compressed":
+            res = experiment_compressed(edb)
         else:
             res = experiment_check256(edb)
         print(json.dumps(res), flush=True)
@@ -603,6 +645,12 @@ def main():
                              "beta_gbs": round(M ** 6 * 8 / (pms * 1e-3) / 1e9, 1),
                              "hbm_frac": round(M ** 6 * 8 / (pms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
         out["speedup_vs_plain_fp64"] = round(value / out["plain_fp64"]["qevals_per_s"], 2)
+        qp.genB_compressed()
+        cms = qp.time_compressed(reps=10)
+        out["plain_fp64_compressed"] = {"qevals_per_s": round(1e3 / cms, 1), "ms_per_qeval": round(cms, 4),
+                                        "table_bytes": qp.compressed_bytes(),
+                                        "stored_beta_bytes": qp.info()["bytes_plain64"]}
+        out["speedup_vs_plain_fp64_compressed"] = round(value / out["plain_fp64_compressed"]["qevals_per_s"], 2)
         qp.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, L)

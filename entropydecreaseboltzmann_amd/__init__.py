@@ -102,8 +102,9 @@ class QCalculator:
         return b.value
 
     def calQ_compressed(self, fin):
-        """calQ with beta formed on the fly from the compressed tables: equal to
-        calQ bit for bit."""
+        """calQ with beta formed on the fly from the compressed tables: every
+        beta equals genB's; the sum runs in another fixed order, so the result
+        equals calQ to FP64 rounding."""
         N = self.N
         f = _c128(fin, (N, N, N))
         q = np.empty_like(f)

@@ -113,9 +113,10 @@ int edb_qcalc_compressed_bytes(const edb_qcalc* q, long* bytes);
 int edb_qcalc_calQ(edb_qcalc* q, double* qout, const double* fin);
 /* calQ with the FP32 copy of beta (edb_qcalc_genB_plain32). */
 int edb_qcalc_calQ_plain32(edb_qcalc* q, double* qout, const double* fin);
-/* calQ with beta formed from the compressed tables (edb_qcalc_genB_compressed):
- * the result of edb_qcalc_calQ, bit for bit, with no M^6 array; host and
- * device arrays as edb_qcalc_calQ / edb_qcalc_calQ_device. */
+/* calQ with beta formed from the compressed tables (edb_qcalc_genB_compressed),
+ * with no M^6 array: every beta equals genB's, the sum runs in another
+ * (fixed) order, so the result equals edb_qcalc_calQ to FP64 rounding.  Host
+ * and device arrays as edb_qcalc_calQ / edb_qcalc_calQ_device. */
 int edb_qcalc_calQ_compressed(edb_qcalc* q, double* qout, const double* fin);
 int edb_qcalc_calQ_compressed_device(edb_qcalc* q, double* d_qout, const double* d_fin, void* stream);
 
