@@ -94,3 +94,23 @@ def test_time_compressed(edb):
     assert q.time_compressed(reps=2) > 0
     with pytest.raises(edb.EdbError):
         q.calQ(rand_spectrum(24, 1))  # the stored array was never built
+
+
+def test_grid_limits(edb):
+    from entropydecreaseboltzmann_amd import _lib
+    with pytest.raises(edb.EdbError) as e:
+        edb.QCalculator(65, 0, 8.0)
+    assert e.value.status == _lib.EDB_EINVAL
+    q = edb.QCalculator(40, 0, 8.0)  # plain paths only past N = 33
+    with pytest.raises(edb.EdbError) as e:
+        q.genB_batched()
+    assert e.value.status == _lib.EDB_EUNSUPPORTED
+    q.closeFilter()
+    q.genB_plain32()  # the stored FP32 array still fits at N = 40 (14 GB)
+    f = rand_spectrum(40, 5)
+    q.genB_compressed()
+    a, b = q.calQ_compressed(f), q.calQ_plain32(f)
+    assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max()  # FP32 beta against FP64 beta
+    with pytest.raises(edb.EdbError) as e:
+        edb.Solver(40, 0, 8.0, False, 1, edb.PREC_FP64_COMPRESSED, "RK4", 0.01, 0.0, edb.FIX_NONE)
+    assert e.value.status == _lib.EDB_EINVAL
