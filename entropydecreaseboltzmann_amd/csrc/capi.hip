@@ -1074,7 +1074,12 @@ int edb_solver_create(const edb_solver_params* p, edb_qcalc* shared, edb_solver*
   EDB_REQUIRE((p->scheme != EDB_SCHEME_CFE && p->scheme != EDB_SCHEME_DUFORT) || p->beta > 0.,
               "solver_create: CFE/DUFORT need beta > 0");
   EDB_REQUIRE(p->history >= 0, "solver_create: history must be >= 0");
-  EDB_REQUIRE(p->N >= 3 && p->N <= kMaxNCalq, "solver_create: N must be in [3, 33]");
+  // the batched kernels exist up to N = 33; the plain precisions (stored or compressed beta, generic
+  // DFT passes past 33) up to 64
+  if (p->precision == EDB_PREC_FP32_BATCHED)
+    EDB_REQUIRE(p->N >= 3 && p->N <= kMaxNCalq, "solver_create: batched precision needs N in [3, 33]");
+  else
+    EDB_REQUIRE(p->N >= 3 && p->N <= kMaxN, "solver_create: N must be in [3, 64]");
   if (shared)
     EDB_REQUIRE(shared->N == p->N && shared->alpha == p->alpha && shared->L == p->L &&
                     shared->filter == (p->filter ? 1 : 0),

@@ -21,8 +21,9 @@
  *     hipStream_t passed as void* (NULL = the library's stream of the handle).
  *   - Handles own their device memory.  A handle must not be used from two
  *     threads at once; different handles may be used concurrently.
- *   - Grids: N in [3, 64] for the plain collision operator (stored or
- *     compressed beta), [3, 33] for the solver and the batched FP32 kernel
+ *   - Grids: N in [3, 64] for the plain collision operator and the solver's
+ *     plain precisions (stored or compressed beta), [3, 33] for the batched
+ *     FP32 kernel
  *     (compiled for N in {3..13, 15, 16, 17, 21, 23, 24, 25, 31, 32, 33}),
  *     any 1 <= Nx, Ny, Nz <= 64 for the transforms.  For odd N the operator
  *     is the reference's; for even N (undefined in the reference) the

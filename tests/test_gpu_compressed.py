@@ -112,5 +112,24 @@ def test_grid_limits(edb):
     a, b = q.calQ_compressed(f), q.calQ_plain32(f)
     assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max()  # FP32 beta against FP64 beta
     with pytest.raises(edb.EdbError) as e:
-        edb.Solver(40, 0, 8.0, False, 1, edb.PREC_FP64_COMPRESSED, "RK4", 0.01, 0.0, edb.FIX_NONE)
+        edb.Solver(40, 0, 8.0, False, 64, edb.PREC_FP32_BATCHED, "RK4", 0.01, 0.0, edb.FIX_NONE)
     assert e.value.status == _lib.EDB_EINVAL
+
+
+def test_solver_past_n33(edb):
+    """An FP64 RK4 solve at N = 40 (generic DFT passes, compressed beta) against
+    the same solve with the stored FP32 beta, and its mass conservation."""
+    N, L, dt = 40, 8.0, 0.01
+    F0 = maxwellian_mixture(N, L, 404)[None]
+    out = {}
+    for prec in (edb.PREC_FP64_COMPRESSED, edb.PREC_FP32_PLAIN):
+        s = edb.Solver(N, 0, L, False, 1, prec, "RK4", dt, 0.0, edb.FIX_NONE)
+        s.set_initial_real(F0)
+        m0 = s.moments()
+        s.step(2)
+        out[prec] = s.get_real()
+        m1 = s.moments()
+        assert abs(m1[0, 0] - m0[0, 0]) <= 1e-12 * abs(m0[0, 0])
+    a, b = out[edb.PREC_FP64_COMPRESSED], out[edb.PREC_FP32_PLAIN]
+    assert np.abs(a - F0).max() > 1e-6 * np.abs(F0).max()  # the state moved
+    assert np.abs(a - b).max() <= 1e-6 * np.abs(a).max()
