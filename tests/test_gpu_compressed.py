@@ -49,12 +49,13 @@ def _negate(N, kcs):
     return ((M - 1 - c1) * M + (M - 1 - c2)) * M + (M - 1 - c3)
 
 
-@pytest.mark.parametrize("N,alpha", [(48, 0), (64, 0), (64, 1)])
-def test_compressed_large_grid_matches_oracle_modes(edb, N, alpha):
+@pytest.mark.parametrize("N,alpha,fil", [(48, 0, 0), (48, 1, 1), (64, 0, 0), (64, 1, 0)])
+def test_compressed_large_grid_matches_oracle_modes(edb, N, alpha, fil):
     L = 8.0
     M = oracle.nmodes(N)
     q = edb.QCalculator(N, alpha, L)
-    q.closeFilter()
+    if not fil:
+        q.closeFilter()
     assert q.info()["bytes_plain64"] > 80e9  # the stored FP64 array would not be practical
     q.genB_compressed()
     f = np.fft.fftn(maxwellian_mixture(N, L, 7 + N))
@@ -62,8 +63,8 @@ def test_compressed_large_grid_matches_oracle_modes(edb, N, alpha):
     rng = np.random.default_rng(N + alpha)
     kcs = np.concatenate([[M ** 3 // 2 + 1], rng.choice(M ** 3, 5, replace=False)])
     kcs = np.concatenate([kcs, _negate(N, kcs)])
-    ref = oracle.calQ_modes(N, f, kcs, alpha, L, False)
-    scale = oracle.calQ_modes(N, f, kcs, alpha, L, False, absval=True).real
+    ref = oracle.calQ_modes(N, f, kcs, alpha, L, fil)
+    scale = oracle.calQ_modes(N, f, kcs, alpha, L, fil, absval=True).real
     got = Q[_centred_to_fftw(N, kcs)]
     assert np.all(np.abs(got - ref) <= 1e-12 * scale), np.abs(got - ref) / scale
     # k = 0: every term has m = -l, where beta is exactly 0
