@@ -65,6 +65,15 @@ class QCalculator {
                                  reinterpret_cast<const double*>(fin)),
           "calQ_batched");
   }
+  // Compressed beta (F over |l+m|^2, |l-m|^2 plus per-mode sigma and Bmm) in
+  // place of genB's M^6 array, and calQ with beta formed from it: every beta
+  // equals genB's, the sum runs in another fixed order (FP64 rounding).  No
+  // reference counterpart; runs for N up to 64.
+  void genBCompressed() { check(edb_qcalc_genB_compressed(h_), "genBCompressed"); }
+  void calQCompressed(c_type* Qout, c_type* fin) {
+    check(edb_qcalc_calQ_compressed(h_, reinterpret_cast<double*>(Qout), reinterpret_cast<const double*>(fin)),
+          "calQCompressed");
+  }
   // Moment correction of every Q (zero discrete mass, momentum and energy
   // moments); no reference counterpart.
   void setConservative(bool on = true) { check(edb_qcalc_set_conservative(h_, on ? 1 : 0), "setConservative"); }

@@ -65,13 +65,17 @@ int main(int argc, char** argv) {
     double b = (double)((x >> 16) % 10);
     v = c_type{a, b};
   }
+  std::vector<c_type> Qc(N);
   {
     edb::QCalculator<c_type, r_type> qc(N0, 0);
     qc.genB();
     qc.calQ(Qout.data(), fin.data());
+    qc.genBCompressed();
+    qc.calQCompressed(Qc.data(), fin.data());
   }
   dump(dir + "/calq_fin.bin", fin.data(), sizeof(c_type) * N);
   dump(dir + "/calq_q.bin", Qout.data(), sizeof(c_type) * N);
+  dump(dir + "/calq_q_compressed.bin", Qc.data(), sizeof(c_type) * N);
 
   // CalQ/testEntropyFix.cpp with N = 1000.
   {

@@ -28,6 +28,8 @@ def test_cpp_mirror_matches_oracle(tmp_path, edb):
     B = oracle.genB(N, 0, 6.62132, True)  # QCalculator(N, 0): filter on, R = 6.62132
     ref = oracle.calQ(N, B, fin)
     assert np.abs(q - ref).max() <= 1e-12 * np.abs(ref).max()
+    qc = ld("calq_q_compressed.bin", np.complex128).reshape(N, N, N)  # calQCompressed, filter on
+    assert np.abs(qc - ref).max() <= 1e-12 * np.abs(ref).max()
     F = ld("fft_F.bin", np.complex128).reshape(8, 8, 8)
     Fh = ld("fft_Fhat.bin", np.complex128).reshape(8, 8, 8)
     Ft = ld("fft_Ftest.bin", np.complex128).reshape(8, 8, 8)
