@@ -133,12 +133,28 @@ int otf_slabs(const edb_qcalc* q) {
   return (int)std::max(1L, std::min(ns, 4096L));
 }
 
-int ensure_plain_scratch(edb_qcalc* q) {
-  if (q->d_dup) return kOk;
-  long W = 2 * q->g.M - 1;
-  EDB_TRY(dmalloc(&q->d_dup, (size_t)(W * W * W * 2)));
-  const long ns = std::max<long>(q->nslab, otf_slabs(q));
-  EDB_TRY(dmalloc(&q->d_part, (size_t)(ns * q->g.NQ * 2)));
+// l-slabs of the plain kernel that takes beta from `src`.
+int plain_slabs(const edb_qcalc* q, int src) {
+  if (src == kPlainCompressed) return otf_slabs(q);
+  if (src == kPlainFP32 && q->nslab32 > 0) return q->nslab32;
+  return q->nslab;
+}
+
+// The difference table and room for `slabs` slab partial sums.  A partial
+// buffer that is too small is retired (kernels in flight may still read it)
+// and replaced.
+int ensure_plain_scratch(edb_qcalc* q, int slabs) {
+  if (!q->d_dup) {
+    long W = 2 * q->g.M - 1;
+    EDB_TRY(dmalloc(&q->d_dup, (size_t)(W * W * W * 2)));
+  }
+  if (q->part_slabs < slabs) {
+    if (q->d_part) q->retired.push_back(q->d_part);
+    q->d_part = nullptr;
+    q->part_slabs = 0;
+    EDB_TRY(dmalloc(&q->d_part, (size_t)slabs * q->g.NQ * 2));
+    q->part_slabs = slabs;
+  }
   return kOk;
 }
 
@@ -149,19 +165,19 @@ int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, i
   if (src == kPlainFP32) EDB_REQUIRE(q->d_plain32, "calQ_plain32: genB_plain32 has not run");
   else if (src == kPlainCompressed) EDB_REQUIRE(q->d_ftab, "calQ_compressed: genB_compressed has not run");
   else EDB_REQUIRE(q->d_plain, "calQ: genB has not run");
-  EDB_TRY(ensure_plain_scratch(q));
+  const int ns = plain_slabs(q, src);
+  EDB_TRY(ensure_plain_scratch(q, ns));
   EDB_TRY(launch_build_dup(q->N, d_fc, fc_stride, q->d_dup, s));
   if (ev0) EDB_HIP_TRY(hipEventRecord(ev0, s));
   if (src == kPlainFP32)
-    EDB_TRY(launch_calq_plain32(q->N, q->d_plain32, q->d_dup, d_fc, fc_stride, q->d_part, q->nslab, s));
+    EDB_TRY(launch_calq_plain32(q->N, q->d_plain32, q->d_dup, d_fc, fc_stride, q->d_part, ns, s));
   else if (src == kPlainCompressed)
     EDB_TRY(launch_calq_plain_otf(q->N, q->tables.rval, q->d_ftab, q->d_sb, q->d_dup, d_fc, fc_stride, q->d_part,
-                                  otf_slabs(q), s));
+                                  ns, s));
   else
-    EDB_TRY(launch_calq_plain64(q->N, q->d_plain, q->d_dup, d_fc, fc_stride, q->d_part, q->nslab, s));
+    EDB_TRY(launch_calq_plain64(q->N, q->d_plain, q->d_dup, d_fc, fc_stride, q->d_part, ns, s));
   if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
-  EDB_TRY(launch_reduce_plain(q->N, q->d_part, src == kPlainCompressed ? otf_slabs(q) : q->nslab, d_q, q_stride,
-                                s));
+  EDB_TRY(launch_reduce_plain(q->N, q->d_part, ns, d_q, q_stride, s));
   return kOk;
 }
 
@@ -560,16 +576,7 @@ int edb_qcalc_tune(edb_qcalc* q, int batched_waves, int plain_slabs) {
   EDB_REQUIRE(plain_slabs >= 0 && plain_slabs <= 4096, "tune: plain_slabs out of range");
   std::lock_guard<std::mutex> lk(q->mu);
   q->nwaves = batched_waves;
-  if (plain_slabs > 0) {
-    int ns = (int)std::min<long>(plain_slabs, q->g.NQ);
-    if (ns != q->nslab) {
-      if (q->d_part) q->retired.push_back(q->d_part);
-      if (q->d_dup) q->retired.push_back(q->d_dup);
-      q->d_part = nullptr;
-      q->d_dup = nullptr;
-      q->nslab = ns;
-    }
-  }
+  if (plain_slabs > 0) q->nslab = (int)std::min<long>(plain_slabs, q->g.NQ);  // d_part grows on use
   return EDB_OK;
 }
 
@@ -640,6 +647,20 @@ int edb_qcalc_autotune(edb_qcalc* q, int batch, int* batched_waves, int* plain_s
       }
     }
     EDB_TRY(edb_qcalc_tune(q, q->nwaves, bs));
+  }
+  if (q->d_plain32) {  // the FP32-beta stream is shorter and wants more slabs in flight
+    double best = 1e30;
+    int bs = 0;
+    for (int waves_per_cu : {16, 32, 48, 64, 96}) {
+      q->nslab32 = std::max(1, (int)std::min<long>({4096L, q->g.NQ, (cu * waves_per_cu + q->ntile - 1) / q->ntile}));
+      double ms = 0.;
+      EDB_TRY(time_plain_src(q, 3, &ms, kPlainFP32));
+      if (ms < best) {
+        best = ms;
+        bs = q->nslab32;
+      }
+    }
+    q->nslab32 = bs;
   }
   if (batched_waves)
     *batched_waves = q->schedule == EDB_SCHED_QUADS ? q->nblocks_quad * quad_waves_per_block(q->N) : q->nwaves;
@@ -1014,7 +1035,7 @@ int prepare_buffers(edb_solver* s) {
     int count = 0;
     EDB_TRY(batched_shape(s->qc, groups, s->p.hermitian ? 1 : 0, &pe, &count));
   } else {
-    EDB_TRY(ensure_plain_scratch(s->qc));
+    EDB_TRY(ensure_plain_scratch(s->qc, plain_slabs(s->qc, plain_source(s))));
   }
   return kOk;
 }

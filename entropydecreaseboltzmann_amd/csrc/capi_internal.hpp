@@ -58,7 +58,9 @@ struct edb_qcalc {
   double* d_ftab = nullptr;
   double* d_sb = nullptr;
   int ntile = 0;
-  int nslab = 16;
+  int nslab = 16;    // l-slabs of the FP64 plain kernel
+  int nslab32 = 0;   // l-slabs of the FP32-beta plain kernel (0: nslab)
+  long part_slabs = 0;  // capacity of d_part, in slabs of M^3 complex
   double* d_dup = nullptr;
   double* d_part = nullptr;
   // paired FP32
@@ -105,7 +107,10 @@ int ensure_moment_table(edb_qcalc* q);
 // double2 [M^3][ld] out); events, if given, bracket the main kernel.
 int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, const float* d_fin,
                    hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-int ensure_plain_scratch(edb_qcalc* q);
+// Plain-path scratch: the difference table and room for `slabs` partial sums.
+int ensure_plain_scratch(edb_qcalc* q, int slabs);
+// l-slabs of the plain kernel for a PlainSource.
+int plain_slabs(const edb_qcalc* q, int src);
 // Plain FP64 Q of one column (centred FP64 input with stride fc_stride).
 // Plain Q of one column with the FP64 beta, or its FP32 copy when fp32.
 // Where the plain kernel takes beta from.
