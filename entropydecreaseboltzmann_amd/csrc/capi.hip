@@ -224,6 +224,8 @@ int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out) {
   // enough l-slabs to give every SIMD several waves of the HBM stream
   q->nslab = std::max(1, std::min(64, (cu_count() * 12 + q->ntile - 1) / q->ntile));
   q->nslab = (int)std::min<long>(q->nslab, q->g.NQ);
+  // the FP32-beta stream is half as long and wants ~4x the waves in flight (measured at N = 24)
+  q->nslab32 = (int)std::max(1L, std::min<long>({4096L, q->g.NQ, (cu_count() * 48L + q->ntile - 1) / q->ntile}));
   // measured default: workgroup quads for the k3-split grids (N >= 27), where
   // they beat stream-K rows by ~10%; rows elsewhere (the autotuner times both)
   q->schedule = batched_ksplit(N) > 1 ? EDB_SCHED_QUADS : EDB_SCHED_ROWS;
