@@ -99,8 +99,9 @@ def traffic_from_profiles():
     return None
 
 
-def profile_kernels(edb, N, batch):
-    """Kernel-only workload for rocprofv3 counter passes."""
+def profile_kernels(edb, N, batch, schedule=None, count=0):
+    """Kernel-only workload for rocprofv3 counter passes (schedule: None for the
+    handle's default, "rows" or "quads" with `count` persistent waves / workgroups)."""
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     n = 256 << 20
@@ -116,6 +117,8 @@ def profile_kernels(edb, N, batch):
     q.genB_batched()
     q.genB()
     q.tune(0, 0)
+    if schedule:
+        q.tune_batched(0 if schedule == "rows" else 1, count)
     tb = q.time_batched(batch, True, reps=2)
     tp = q.time_plain(reps=2)
     print(json.dumps({"profile_kernels": True, "N": N, "batch": batch, "batched_ms": tb,
@@ -540,6 +543,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--experiment", choices=["bkw16", "bkw24", "n32", "check256", "compressed"],
                     help="run one of the reference configurations with its error report")
+    ap.add_argument("--schedule", choices=["rows", "quads"],
+                    help="batched work schedule for --profile-kernels (default: the handle's)")
+    ap.add_argument("--count", type=int, default=0,
+                    help="persistent waves (rows) or workgroups (quads) for --schedule; 0 = default")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="no solver: one 256 MiB device copy (the reference transfer for PMC "
                          "scaling), then 3 batched and 3 plain Q launches at --N/--batch")
@@ -552,7 +559,7 @@ def main():
     import entropydecreaseboltzmann_amd as edb
 
     if args.profile_kernels:
-        profile_kernels(edb, args.N, args.batch)
+        profile_kernels(edb, args.N, args.batch, args.schedule, args.count)
         return
     if args.serve:
         serve(edb, args)
