@@ -46,6 +46,13 @@ def test_invalid_arguments_rejected_before_device_use():
     assert lib.edb_solver_create(ctypes.byref(p), None, ctypes.byref(h)) == _lib.EDB_EINVAL
     p.scheme, p.beta = 4, 0.0  # CFE needs beta > 0
     assert lib.edb_solver_create(ctypes.byref(p), None, ctypes.byref(h)) == _lib.EDB_EINVAL
+    assert lib.edb_qcalc_create(65, 0, 8.0, ctypes.byref(h)) == _lib.EDB_EINVAL  # plain paths stop at 64
+    p.scheme, p.beta, p.N = 3, 0.0, 40
+    p.precision = _lib.PREC_FP32_BATCHED  # the batched kernels stop at N = 33
+    assert lib.edb_solver_create(ctypes.byref(p), None, ctypes.byref(h)) == _lib.EDB_EINVAL
+    assert b"batched precision" in lib.edb_last_error()
+    p.N, p.precision = 9, 4  # no such precision
+    assert lib.edb_solver_create(ctypes.byref(p), None, ctypes.byref(h)) == _lib.EDB_EINVAL
     st = _lib.EfixState()
     assert lib.edb_efix_init(ctypes.byref(st), 0, 1e-10) == _lib.EDB_EINVAL
     assert lib.edb_efix_init(ctypes.byref(st), 10, 1e-10) == _lib.EDB_OK
