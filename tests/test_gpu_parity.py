@@ -182,6 +182,43 @@ def test_largest_grid_batched_matches_plain(edb, schedule):
         assert np.abs(got[i][0, 0, :] - rev[0, 0, :]).max() <= 1e-6 * np.abs(ref).max()
 
 
+COMPILED_BATCHED = [3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 15, 16, 17, 21, 23, 24, 25, 31, 32, 33]
+
+
+@pytest.mark.parametrize("N", COMPILED_BATCHED)
+def test_batched_every_compiled_size_matches_plain(edb, N):
+    """Every compiled batched kernel (default schedule, Hermitian half, 70
+    instances: two instance groups, the second partial) against the FP64 plain
+    path on the GPU, with alpha and the filter alternating over the sizes, to
+    the FP32 bar 3e-6 S_k, S_k = sum_l |beta||f_l||f_[k-l]| from the oracle's
+    per-mode sum, at the 8 modes with the largest difference and 8 random ones."""
+    L, batch = 8.0, 70
+    alpha, fil = N % 2, (N // 2) % 2
+    M, n = oracle.nmodes(N), (N - 1) // 2
+    fs = hermitize(np.stack([np.fft.fftn(maxwellian_mixture(N, L, 500 + N + i)) for i in range(batch)]))
+    q = edb.QCalculator(N, alpha, L)
+    if not fil:
+        q.closeFilter()
+    q.genB_batched()
+    q.genB()
+    got = q.calQ_batched(fs, hermitian=True)
+    # centred mode index of each FFTW-order grid point (-1 on the Nyquist planes of even N)
+    w = np.array([(f if f <= n else f - N) for f in range(N)]) + n
+    valid = (w >= 0) & (w < M)
+    cen = np.where(valid[:, None, None] & valid[None, :, None] & valid[None, None, :],
+                   (w[:, None, None] * M + w[None, :, None]) * M + w[None, None, :], -1).ravel()
+    rng = np.random.default_rng(N)
+    for i in (0, 63, 64, batch - 1):
+        ref = q.calQ(fs[i])
+        err = np.abs(got[i] - ref).ravel()
+        modes = np.flatnonzero(cen >= 0)
+        worst = modes[np.argsort(err[modes])[-8:]]
+        pick = np.concatenate([worst, rng.choice(modes, 8, replace=False)])
+        S = oracle.calQ_modes(N, fs[i], cen[pick], alpha, L, fil, absval=True).real
+        assert np.all(err[pick] <= 3e-6 * S), (N, i, (err[pick] / S).max())
+        assert got[i][0, 0, 0] == 0.0
+
+
 def test_plain_and_batched_timers(edb):
     q = edb.QCalculator(9, 0, 8.0)
     q.genB()
