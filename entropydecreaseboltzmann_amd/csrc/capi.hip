@@ -181,6 +181,20 @@ int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, i
   return kOk;
 }
 
+// Runs one genB build; when any step of it fails, drains the stream and frees
+// every buffer that build owns, so a later call starts again from nothing
+// instead of finding a half-built (or never-filled) array and reporting success.
+template <typename Build, typename Release>
+int build_or_release(edb_qcalc* q, Build build, Release release) {
+  const int st = build();
+  if (st != EDB_OK) {
+    (void)hipStreamSynchronize(q->stream);
+    (void)hipGetLastError();
+    release();
+  }
+  return st;
+}
+
 }  // namespace edb
 
 extern "C" {
@@ -254,36 +268,54 @@ int edb_qcalc_genB(edb_qcalc* q) {
   EDB_REQUIRE(q, "null handle");
   std::lock_guard<std::mutex> lk(q->mu);
   if (q->d_plain) return EDB_OK;
-  EDB_TRY(ensure_tables(q));
-  EDB_TRY(dmalloc(&q->d_plain, (size_t)q->ntile * q->g.NQ * 64));
-  EDB_TRY(launch_genb_plain64(q->tables, q->d_plain, q->ntile, q->stream));
-  EDB_HIP_TRY(hipStreamSynchronize(q->stream));
-  return EDB_OK;
+  return build_or_release(
+      q,
+      [&]() -> int {
+        EDB_TRY(ensure_tables(q));
+        EDB_TRY(dmalloc(&q->d_plain, (size_t)q->ntile * q->g.NQ * 64));
+        EDB_TRY(launch_genb_plain64(q->tables, q->d_plain, q->ntile, q->stream));
+        EDB_HIP_TRY(hipStreamSynchronize(q->stream));
+        return EDB_OK;
+      },
+      [&] { dfree(&q->d_plain); });
 }
 
 int edb_qcalc_genB_plain32(edb_qcalc* q) {
   EDB_REQUIRE(q, "null handle");
   std::lock_guard<std::mutex> lk(q->mu);
   if (q->d_plain32) return EDB_OK;
-  EDB_TRY(ensure_tables(q));
-  EDB_TRY(dmalloc(&q->d_plain32, (size_t)q->ntile * q->g.NQ * 64));
-  EDB_TRY(launch_genb_plain32(q->tables, q->d_plain32, q->ntile, q->stream));
-  EDB_HIP_TRY(hipStreamSynchronize(q->stream));
-  return EDB_OK;
+  return build_or_release(
+      q,
+      [&]() -> int {
+        EDB_TRY(ensure_tables(q));
+        EDB_TRY(dmalloc(&q->d_plain32, (size_t)q->ntile * q->g.NQ * 64));
+        EDB_TRY(launch_genb_plain32(q->tables, q->d_plain32, q->ntile, q->stream));
+        EDB_HIP_TRY(hipStreamSynchronize(q->stream));
+        return EDB_OK;
+      },
+      [&] { dfree(&q->d_plain32); });
 }
 
 int edb_qcalc_genB_compressed(edb_qcalc* q) {
   EDB_REQUIRE(q, "null handle");
   std::lock_guard<std::mutex> lk(q->mu);
-  if (q->d_ftab) return EDB_OK;
-  EDB_TRY(ensure_tables(q));
-  long R, Rh;
-  compressed_beta_dims(q->N, &R, &Rh);
-  EDB_TRY(dmalloc(&q->d_ftab, (size_t)(R * Rh)));
-  EDB_TRY(dmalloc(&q->d_sb, (size_t)q->g.NQ * 2));
-  EDB_TRY(launch_genb_compressed(q->tables, q->d_ftab, q->d_sb, q->stream));
-  EDB_HIP_TRY(hipStreamSynchronize(q->stream));
-  return EDB_OK;
+  if (q->d_ftab) return EDB_OK;  // set only by a build that finished
+  return build_or_release(
+      q,
+      [&]() -> int {
+        EDB_TRY(ensure_tables(q));
+        long R, Rh;
+        compressed_beta_dims(q->N, &R, &Rh);
+        EDB_TRY(dmalloc(&q->d_ftab, (size_t)(R * Rh)));
+        EDB_TRY(dmalloc(&q->d_sb, (size_t)q->g.NQ * 2));
+        EDB_TRY(launch_genb_compressed(q->tables, q->d_ftab, q->d_sb, q->stream));
+        EDB_HIP_TRY(hipStreamSynchronize(q->stream));
+        return EDB_OK;
+      },
+      [&] {
+        dfree(&q->d_ftab);
+        dfree(&q->d_sb);
+      });
 }
 
 int edb_qcalc_compressed_bytes(const edb_qcalc* q, long* bytes) {
@@ -297,35 +329,57 @@ int edb_qcalc_compressed_bytes(const edb_qcalc* q, long* bytes) {
 int edb_qcalc_genB_batched(edb_qcalc* q) {
   EDB_REQUIRE(q, "null handle");
   std::lock_guard<std::mutex> lk(q->mu);
-  if (q->d_pair) return EDB_OK;
+  if (q->d_pair) return EDB_OK;  // set only by a build that finished
   if (!batched_supported(q->N)) {
     set_error("genB_batched: no compiled batched kernel for N = " + std::to_string(q->N));
     return EDB_EUNSUPPORTED;
   }
-  EDB_TRY(ensure_tables(q));
-  std::vector<uint32_t> sched;
-  build_pair_plan(q->N, &q->rowinfo, &sched, &q->nrows_half);
-  q->nblocks = (long)sched.size();
-  EDB_TRY(dmalloc(&q->d_rowinfo, q->rowinfo.size()));
-  EDB_TRY(dmalloc(&q->d_sched, sched.size()));
-  EDB_HIP_TRY(hipMemcpy(q->d_rowinfo, q->rowinfo.data(), sizeof(int4) * q->rowinfo.size(),
-                        hipMemcpyHostToDevice));
-  EDB_HIP_TRY(hipMemcpy(q->d_sched, sched.data(), sizeof(uint32_t) * sched.size(), hipMemcpyHostToDevice));
-  std::vector<int> rowidx, ublk;
-  EDB_TRY(build_quad_plan(q->N, q->rowinfo, sched, q->nrows_half, &q->quadinfo, &rowidx, &ublk,
-                          &q->nquads_half));
-  EDB_TRY(dmalloc(&q->d_quadinfo, q->quadinfo.size()));
-  EDB_TRY(dmalloc(&q->d_rowidx, rowidx.size()));
-  EDB_TRY(dmalloc(&q->d_ublk, ublk.size()));
-  EDB_HIP_TRY(hipMemcpy(q->d_quadinfo, q->quadinfo.data(), sizeof(int4) * q->quadinfo.size(),
-                        hipMemcpyHostToDevice));
-  EDB_HIP_TRY(hipMemcpy(q->d_rowidx, rowidx.data(), sizeof(int) * rowidx.size(), hipMemcpyHostToDevice));
-  EDB_HIP_TRY(hipMemcpy(q->d_ublk, ublk.data(), sizeof(int) * ublk.size(), hipMemcpyHostToDevice));
-  EDB_TRY(dmalloc(&q->d_pair, (size_t)q->nblocks * q->g.M * q->g.M));
-  EDB_TRY(launch_genb_pair32(q->tables, (const int*)q->d_rowinfo, q->d_sched, (int)q->rowinfo.size(),
-                             q->nblocks, q->d_pair, q->stream));
-  EDB_HIP_TRY(hipStreamSynchronize(q->stream));
-  return EDB_OK;
+  float* pair = nullptr;
+  return build_or_release(
+      q,
+      [&]() -> int {
+        EDB_TRY(ensure_tables(q));
+        std::vector<uint32_t> sched;
+        build_pair_plan(q->N, &q->rowinfo, &sched, &q->nrows_half);
+        q->nblocks = (long)sched.size();
+        EDB_TRY(dmalloc(&q->d_rowinfo, q->rowinfo.size()));
+        EDB_TRY(dmalloc(&q->d_sched, sched.size()));
+        EDB_HIP_TRY(hipMemcpy(q->d_rowinfo, q->rowinfo.data(), sizeof(int4) * q->rowinfo.size(),
+                              hipMemcpyHostToDevice));
+        EDB_HIP_TRY(
+            hipMemcpy(q->d_sched, sched.data(), sizeof(uint32_t) * sched.size(), hipMemcpyHostToDevice));
+        std::vector<int> rowidx, ublk;
+        EDB_TRY(build_quad_plan(q->N, q->rowinfo, sched, q->nrows_half, &q->quadinfo, &rowidx, &ublk,
+                                &q->nquads_half));
+        EDB_TRY(dmalloc(&q->d_quadinfo, q->quadinfo.size()));
+        EDB_TRY(dmalloc(&q->d_rowidx, rowidx.size()));
+        EDB_TRY(dmalloc(&q->d_ublk, ublk.size()));
+        EDB_HIP_TRY(hipMemcpy(q->d_quadinfo, q->quadinfo.data(), sizeof(int4) * q->quadinfo.size(),
+                              hipMemcpyHostToDevice));
+        EDB_HIP_TRY(
+            hipMemcpy(q->d_rowidx, rowidx.data(), sizeof(int) * rowidx.size(), hipMemcpyHostToDevice));
+        EDB_HIP_TRY(hipMemcpy(q->d_ublk, ublk.data(), sizeof(int) * ublk.size(), hipMemcpyHostToDevice));
+        // the paired array is published in d_pair only once it is filled
+        EDB_TRY(dmalloc(&pair, (size_t)q->nblocks * q->g.M * q->g.M));
+        EDB_TRY(launch_genb_pair32(q->tables, (const int*)q->d_rowinfo, q->d_sched,
+                                   (int)q->rowinfo.size(), q->nblocks, pair, q->stream));
+        EDB_HIP_TRY(hipStreamSynchronize(q->stream));
+        q->d_pair = pair;
+        return EDB_OK;
+      },
+      [&] {
+        dfree(&pair);
+        dfree(&q->d_rowinfo);
+        dfree(&q->d_sched);
+        dfree(&q->d_quadinfo);
+        dfree(&q->d_rowidx);
+        dfree(&q->d_ublk);
+        q->rowinfo.clear();
+        q->quadinfo.clear();
+        q->nblocks = 0;
+        q->nrows_half = 0;
+        q->nquads_half = 0;
+      });
 }
 
 namespace {

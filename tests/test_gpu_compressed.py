@@ -117,6 +117,25 @@ def test_grid_limits(edb):
     assert e.value.status == _lib.EDB_EINVAL
 
 
+def test_failed_genB_leaves_nothing_built(edb):
+    """The stored FP64 array at N = 64 (500 GB) cannot be allocated: genB
+    reports EDB_ENOMEM on every call (a retry does not find a half-built array
+    and report success), calQ refuses to run, and the handle still builds and
+    runs the compressed beta."""
+    from entropydecreaseboltzmann_amd import _lib
+    q = edb.QCalculator(64, 0, 8.0)
+    assert q.info()["bytes_plain64"] > 400e9
+    for _ in range(2):
+        with pytest.raises(edb.EdbError) as e:
+            q.genB()
+        assert e.value.status == _lib.EDB_ENOMEM
+    f = rand_spectrum(64, 3)
+    with pytest.raises(edb.EdbError):
+        q.calQ(f)
+    q.genB_compressed()
+    assert np.isfinite(q.calQ_compressed(f)).all()
+
+
 def test_solver_past_n33(edb):
     """An FP64 RK4 solve at N = 40 (generic DFT passes, compressed beta) against
     the same solve with the stored FP32 beta, and its mass conservation."""
