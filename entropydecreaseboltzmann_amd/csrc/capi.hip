@@ -60,7 +60,11 @@ int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out) {
     e.nitems = (int)pre.size() - 1;
     e.total = pre.back();
     EDB_TRY(dmalloc(&e.d_prefix, pre.size()));
-    EDB_HIP_TRY(hipMemcpy(e.d_prefix, pre.data(), sizeof(long) * pre.size(), hipMemcpyHostToDevice));
+    if (hipMemcpy(e.d_prefix, pre.data(), sizeof(long) * pre.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("batched prefix: copy to the device failed");
+      dfree(&e.d_prefix);
+      return kHipError;
+    }
     it = q->prefixes.emplace(key, e).first;
   }
   *out = &it->second;
@@ -83,8 +87,14 @@ int ensure_moment_table(edb_qcalc* q) {
   std::vector<double> tab;
   int E = 0;
   EDB_TRY(build_moment_table(q->N, q->L, &tab, &E));
-  EDB_TRY(dmalloc(&q->d_mom, tab.size()));
-  EDB_HIP_TRY(hipMemcpy(q->d_mom, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+  double* d = nullptr;
+  EDB_TRY(dmalloc(&d, tab.size()));
+  if (hipMemcpy(d, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("moment table: copy to the device failed");
+    dfree(&d);
+    return kHipError;
+  }
+  q->d_mom = d;  // published only once filled
   q->mom_E = E;
   return kOk;
 }

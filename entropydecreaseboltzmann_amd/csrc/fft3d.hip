@@ -165,7 +165,11 @@ int twiddles(int NA, int sign, const double** out) {
   }
   double* d = nullptr;
   EDB_HIP_TRY(hipMalloc(&d, sizeof(double) * 2 * NA));
-  EDB_HIP_TRY(hipMemcpy(d, h.data(), sizeof(double) * 2 * NA, hipMemcpyHostToDevice));
+  if (hipMemcpy(d, h.data(), sizeof(double) * 2 * NA, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("fft3d: twiddle copy to the device failed");
+    (void)hipFree(d);
+    return kHipError;
+  }
   g_tw[key] = d;
   if (out) *out = d;
   return kOk;

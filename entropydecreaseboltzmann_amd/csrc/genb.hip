@@ -205,10 +205,17 @@ int beta_tables_init(BetaTables* t, int N, int alpha, double L, int filter) {
         long A = 4L * (k1 * k1 + k2 * k2 + k3 * k3);
         bmm[i++] = funF_lattice(alpha, A, 0, t->lambda) * fil * fil * t->rval;
       }
-  EDB_HIP_TRY(hipMalloc(&t->d_sig, sizeof(double) * g.M));
-  EDB_HIP_TRY(hipMalloc(&t->d_bmm, sizeof(double) * g.NQ));
-  EDB_HIP_TRY(hipMemcpy(t->d_sig, sig.data(), sizeof(double) * g.M, hipMemcpyHostToDevice));
-  EDB_HIP_TRY(hipMemcpy(t->d_bmm, bmm.data(), sizeof(double) * g.NQ, hipMemcpyHostToDevice));
+  t->d_sig = t->d_bmm = nullptr;
+  hipError_t e = hipMalloc(&t->d_sig, sizeof(double) * g.M);
+  if (e == hipSuccess) e = hipMalloc(&t->d_bmm, sizeof(double) * g.NQ);
+  if (e == hipSuccess) e = hipMemcpy(t->d_sig, sig.data(), sizeof(double) * g.M, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t->d_bmm, bmm.data(), sizeof(double) * g.NQ, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {  // nothing half-built is kept
+    set_error(std::string("beta tables: ") + hipGetErrorString(e));
+    (void)hipGetLastError();
+    beta_tables_free(t);
+    return e == hipErrorOutOfMemory ? kNoMemory : kHipError;
+  }
   return kOk;
 }
 
