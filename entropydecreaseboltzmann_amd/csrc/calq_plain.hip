@@ -16,6 +16,9 @@
 // D[d] = f_[d]_N or 0 on the Nyquist residue, so the lane's address is its
 // own base minus a uniform offset per l.  Slabs of l are summed afterwards in
 // a fixed order (deterministic).
+#include <algorithm>
+#include <cstdlib>
+
 #include "edb_internal.hpp"
 
 namespace edb {
@@ -103,19 +106,30 @@ __device__ __forceinline__ int wrap_small(int N, int n, int d, bool* ok) {
 // The plain evaluation with beta formed on the fly from the compressed tables
 // (genb.hip): beta(l, m) = T[|l+m|^2 Rh + |l-m|^2 / 2] sigma_l sigma_m rval -
 // Bmm[m], in genb's order, so every beta equals the stored one bit for bit.
-// Nothing of size M^6 exists.
+// Nothing of size M^6 exists.  Lanes run over 64 consecutive l, so the f_l,
+// f_[k-l] (difference table) and (sigma, Bmm)[m] reads are contiguous across
+// lanes (m = k - l runs backwards with l); each lane accumulates its l terms
+// for the wave's K modes in l order, the 64 lanes are then summed with a fixed
+// butterfly and slabs of l are summed afterwards in a fixed order, so the
+// result is deterministic (it differs from the stored kernel only in
+// summation order).  Two ways of choosing a wave's K modes:
 //
-// Lanes run over 64 consecutive l and a wave owns K consecutive k3 of one
-// k-row (k1, k2), wave-uniform: |l+m|^2 = |k - N w|^2 (w the wrap of k - l)
-// depends on k alone wherever no wrap occurs, so the lanes of a wave read one
-// row of T and the row stays in L1 across the wave's sweep of l; the f_[k-l]
-// and (sigma, Bmm)[m] reads are contiguous across lanes (m = k - l runs
-// backwards with l).  Each lane accumulates its l terms for the K modes in
-// l order; the 64 lanes are then summed with a fixed butterfly, and slabs of
-// l are summed afterwards in a fixed order, so the result is deterministic
-// (it differs from the stored kernel only in summation order).
+// k3 block (grids up to M = 27): K consecutive k3 of one k-row (k1, k2).  The
+// K modes' f_[k-l] and (sigma, Bmm) reads of a lane fall on the same cache
+// lines, and T (12 MB at N = 25) stays in L2 whichever rows are in use.
+//
+// Orbit (M >= 29).  For a mode k, l + m = k - N w where w is the wrap of
+// k - l, so the row |l+m|^2 depends only on which axes wrap and on |k1|,
+// |k2|, |k3|: the modes of one orbit of the cube's symmetries (sign changes
+// and axis permutations) read the same <= 8 rows of T.  A wave takes K modes
+// of one orbit (the groups come in orbit order) and a slab of l, and block b
+// runs item (b mod 8) * per_xcd + b / 8: workgroups are dealt to the eight
+// XCDs in turn, so each XCD walks its own contiguous run of orbits and the
+// rows its waves share stay in its L2 (at N = 64 a row is 46 KB and T
+// 536 MB).  The price is K separate lines per lane for the f_[k-l] and
+// (sigma, Bmm) reads, which costs more than it saves while T fits in L2.
 template <int K>
-__global__ __launch_bounds__(64) void calq_plain_otf_kernel(int N, int M, long NQ, double rval,
+__global__ __launch_bounds__(64) void calq_plain_otf_k3_kernel(int N, int M, long NQ, double rval,
                                                             const double* __restrict__ T, int Rh,
                                                             const double2* __restrict__ SB,
                                                             const double2* __restrict__ dup,
@@ -180,6 +194,77 @@ __global__ __launch_bounds__(64) void calq_plain_otf_kernel(int N, int M, long N
   }
 }
 
+template <int K>
+__global__ __launch_bounds__(64) void calq_plain_otf_orbit_kernel(int N, int M, long NQ, double rval,
+                                                            const double* __restrict__ T, int Rh,
+                                                            const double2* __restrict__ SB,
+                                                            const double2* __restrict__ dup,
+                                                            const double2* __restrict__ fc, int stride,
+                                                            const int* __restrict__ kgrp, long nitems,
+                                                            long per_xcd, int nslab,
+                                                            double2* __restrict__ part) {
+  const int n = (N - 1) / 2, W = 2 * M - 1;
+  const int lane = threadIdx.x;
+  const long b = blockIdx.x;
+  const long item = (b & 7) * per_xcd + (b >> 3);
+  if (item >= nitems) return;
+  const long grp = item / nslab;
+  const int slab = (int)(item - grp * nslab);
+  int kc[K], k1[K], k2[K], k3[K], dk[K];
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    const int c = __builtin_amdgcn_readfirstlane(kgrp[grp * K + u]);
+    kc[u] = c;
+    const int cc = c >= 0 ? c : 0;  // padding entries address mode 0 and add nothing
+    k1[u] = cc / (M * M);
+    k2[u] = (cc / M) % M;
+    k3[u] = cc % M;
+    dk[u] = ((k1[u] + M - 1) * W + (k2[u] + M - 1)) * W + (k3[u] + M - 1);
+  }
+  const int l0 = (int)(NQ * slab / nslab), l1e = (int)(NQ * (slab + 1) / nslab);
+  double ar[K], ai[K];
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    ar[u] = 0.;
+    ai[u] = 0.;
+  }
+  for (int lb = l0; lb < l1e; lb += 64) {
+    const bool active = lb + lane < l1e;
+    const int l = active ? lb + lane : l1e - 1;  // inactive lanes read valid addresses, add 0
+    const int row = l / M, a3 = l - row * M;
+    const int a1 = row / M, a2 = row - a1 * M;
+    const double2 fl = fc[(long)l * stride];
+    const double sl = SB[l].x;
+    const int l1 = a1 - n, l2 = a2 - n, l3 = a3 - n;
+    const int ld = (a1 * W + a2) * W + a3;  // f_[k-l] sits at dk - ld
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+#pragma clang fp contract(off)
+      bool v1, v2, v3;
+      const int m1 = wrap_small(N, n, k1[u] - a1, &v1), m2 = wrap_small(N, n, k2[u] - a2, &v2),
+                m3 = wrap_small(N, n, k3[u] - a3, &v3);
+      const int A = (l1 + m1) * (l1 + m1) + (l2 + m2) * (l2 + m2) + (l3 + m3) * (l3 + m3);
+      const int Bq = (l1 - m1) * (l1 - m1) + (l2 - m2) * (l2 - m2) + (l3 - m3) * (l3 - m3);
+      const double F = T[A * Rh + (Bq >> 1)];
+      const double2 sb = SB[((m1 + n) * M + (m2 + n)) * M + (m3 + n)];
+      const double2 c = dup[dk[u] - ld];
+      const double v = F * sl * sb.x * rval - sb.y;
+      const double bt = (kc[u] >= 0 && active && v1 && v2 && v3 && A != 0) ? v : 0.;
+      accum_term(fl, c, bt, ar[u], ai[u]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    double r = ar[u], i = ai[u];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      r += __shfl_xor(r, o);
+      i += __shfl_xor(i, o);
+    }
+    if (lane == 0 && kc[u] >= 0) part[(long)slab * NQ + kc[u]] = make_double2(r, i);
+  }
+}
+
 __global__ void reduce_plain_kernel(long NQ, const double2* __restrict__ part, int nslab,
                                     double2* __restrict__ q, int stride) {
   long kc = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -228,16 +313,53 @@ int launch_calq_plain32(int N, const float* d_B, const double* d_dup, const doub
   return kOk;
 }
 
+int build_orbit_groups(int N, std::vector<int>* out) {
+  Geometry g(N);
+  struct Mode {
+    long key;  // |k|^2, then |k1|, |k2|, |k3| in decreasing order: one key per orbit
+    int kc;
+  };
+  std::vector<Mode> modes;
+  modes.reserve(g.NQ);
+  for (int c = 0; c < g.NQ; ++c) {
+    int a[3] = {std::abs(c / (g.M * g.M) - g.n), std::abs((c / g.M) % g.M - g.n), std::abs(c % g.M - g.n)};
+    std::sort(a, a + 3);
+    const long r2 = (long)a[0] * a[0] + (long)a[1] * a[1] + (long)a[2] * a[2];
+    modes.push_back({((r2 * 64 + a[2]) * 64 + a[1]) * 64 + a[0], c});
+  }
+  std::sort(modes.begin(), modes.end(), [](const Mode& x, const Mode& y) {
+    return x.key != y.key ? x.key < y.key : x.kc < y.kc;
+  });
+  out->clear();
+  for (size_t i = 0; i < modes.size();) {
+    size_t j = i;
+    while (j < modes.size() && modes[j].key == modes[i].key) out->push_back(modes[j++].kc);
+    while (out->size() % kOtfK) out->push_back(-1);
+    i = j;
+  }
+  return (int)(out->size() / kOtfK);
+}
+
 int launch_calq_plain_otf(int N, double rval, const double* d_T, const double* d_SB, const double* d_dup,
-                          const double* d_fc, int fc_stride, double* d_part, int nslab, hipStream_t s) {
+                          const double* d_fc, int fc_stride, const int* d_kgrp, int ngroups, double* d_part,
+                          int nslab, hipStream_t s) {
   Geometry g(N);
   EDB_REQUIRE(nslab >= 1 && nslab <= 4096 && nslab <= g.NQ, "calq_plain_otf: bad slab count");
   long R, Rh;
   compressed_beta_dims(N, &R, &Rh);
-  const int nkb = g.M * g.M * ((g.M + kOtfK - 1) / kOtfK);
-  hipLaunchKernelGGL((calq_plain_otf_kernel<kOtfK>), dim3(nkb, nslab), dim3(64), 0, s, N, g.M, g.NQ, rval, d_T,
-                     (int)Rh, (const double2*)d_SB, (const double2*)d_dup, (const double2*)d_fc, fc_stride,
-                     (double2*)d_part, nslab);
+  if (!otf_orbit_layout(N)) {
+    const int nkb = g.M * g.M * ((g.M + kOtfK - 1) / kOtfK);
+    hipLaunchKernelGGL((calq_plain_otf_k3_kernel<kOtfK>), dim3(nkb, nslab), dim3(64), 0, s, N, g.M, g.NQ, rval,
+                       d_T, (int)Rh, (const double2*)d_SB, (const double2*)d_dup, (const double2*)d_fc, fc_stride,
+                       (double2*)d_part, nslab);
+  } else {
+    EDB_REQUIRE(d_kgrp && ngroups >= 1, "calq_plain_otf: no mode groups");
+    const long nitems = (long)ngroups * nslab, per_xcd = (nitems + 7) / 8;
+    EDB_REQUIRE(8 * per_xcd <= 0x7fffffffL, "calq_plain_otf: grid too large");
+    hipLaunchKernelGGL((calq_plain_otf_orbit_kernel<kOtfK>), dim3((unsigned)(8 * per_xcd)), dim3(64), 0, s, N,
+                       g.M, g.NQ, rval, d_T, (int)Rh, (const double2*)d_SB, (const double2*)d_dup,
+                       (const double2*)d_fc, fc_stride, d_kgrp, nitems, per_xcd, nslab, (double2*)d_part);
+  }
   EDB_HIP_TRY(hipGetLastError());
   return kOk;
 }

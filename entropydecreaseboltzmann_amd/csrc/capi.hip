@@ -134,11 +134,18 @@ int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, 
                                   count, s, ev0, ev1);
 }
 
-// l-slabs of the on-the-fly plain kernel: about eight waves per SIMD, each
-// slab at least 1024 l.
+// l-slabs of the on-the-fly plain kernel, each at least 1024 l.  k3-block
+// layout: about eight waves per SIMD.  Orbit layout: up to 32, so that many
+// waves work on each orbit's rows of F at once (the waves of one XCD then span
+// a few orbits), at the cost of 16 bytes per mode and slab of partial sums.
 int otf_slabs(const edb_qcalc* q) {
-  const long nkb = (long)q->g.M * q->g.M * ((q->g.M + kOtfK - 1) / kOtfK);
-  long ns = (cu_count() * 32L + nkb - 1) / nkb;
+  long ns;
+  if (otf_orbit_layout(q->N)) {
+    ns = 32;
+  } else {
+    const long nkb = (long)q->g.M * q->g.M * ((q->g.M + kOtfK - 1) / kOtfK);
+    ns = (cu_count() * 32L + nkb - 1) / nkb;
+  }
   ns = std::min(ns, std::max(1L, q->g.NQ / 1024));
   return (int)std::max(1L, std::min(ns, 4096L));
 }
@@ -182,8 +189,8 @@ int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, i
   if (src == kPlainFP32)
     EDB_TRY(launch_calq_plain32(q->N, q->d_plain32, q->d_dup, d_fc, fc_stride, q->d_part, ns, s));
   else if (src == kPlainCompressed)
-    EDB_TRY(launch_calq_plain_otf(q->N, q->tables.rval, q->d_ftab, q->d_sb, q->d_dup, d_fc, fc_stride, q->d_part,
-                                  ns, s));
+    EDB_TRY(launch_calq_plain_otf(q->N, q->tables.rval, q->d_ftab, q->d_sb, q->d_dup, d_fc, fc_stride, q->d_kgrp,
+                                  q->n_kgrp, q->d_part, ns, s));
   else
     EDB_TRY(launch_calq_plain64(q->N, q->d_plain, q->d_dup, d_fc, fc_stride, q->d_part, ns, s));
   if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
@@ -316,8 +323,13 @@ int edb_qcalc_genB_compressed(edb_qcalc* q) {
         EDB_TRY(ensure_tables(q));
         long R, Rh;
         compressed_beta_dims(q->N, &R, &Rh);
+        std::vector<int> grp;
+        const int ngroups = build_orbit_groups(q->N, &grp);
         EDB_TRY(dmalloc(&q->d_ftab, (size_t)(R * Rh)));
         EDB_TRY(dmalloc(&q->d_sb, (size_t)q->g.NQ * 2));
+        EDB_TRY(dmalloc(&q->d_kgrp, grp.size()));
+        EDB_HIP_TRY(hipMemcpy(q->d_kgrp, grp.data(), sizeof(int) * grp.size(), hipMemcpyHostToDevice));
+        q->n_kgrp = ngroups;
         EDB_TRY(launch_genb_compressed(q->tables, q->d_ftab, q->d_sb, q->stream));
         EDB_HIP_TRY(hipStreamSynchronize(q->stream));
         return EDB_OK;
@@ -325,6 +337,8 @@ int edb_qcalc_genB_compressed(edb_qcalc* q) {
       [&] {
         dfree(&q->d_ftab);
         dfree(&q->d_sb);
+        dfree(&q->d_kgrp);
+        q->n_kgrp = 0;
       });
 }
 
@@ -749,6 +763,7 @@ int edb_qcalc_destroy(edb_qcalc* q) {
   dfree(&q->d_plain32);
   dfree(&q->d_ftab);
   dfree(&q->d_sb);
+  dfree(&q->d_kgrp);
   dfree(&q->d_plain);
   dfree(&q->d_dup);
   dfree(&q->d_part);

@@ -57,6 +57,8 @@ struct edb_qcalc {
   // compressed beta: F table over (|l+m|^2, |l-m|^2 / 2) and (sigma, Bmm) per mode
   double* d_ftab = nullptr;
   double* d_sb = nullptr;
+  int* d_kgrp = nullptr;  // modes in groups of kOtfK from one symmetry orbit (-1 pads)
+  int n_kgrp = 0;         // number of groups
   int ntile = 0;
   int nslab = 16;    // l-slabs of the FP64 plain kernel
   int nslab32 = 0;   // l-slabs of the FP32-beta plain kernel (0: nslab)

@@ -16,6 +16,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace edb {
 
@@ -102,12 +103,19 @@ int launch_genb_pair32(const BetaTables& t, const int* d_rowinfo, const uint32_t
 // Compressed beta: the F table (R x Rh doubles) and the per-mode
 // (sigma product, Bmm) pairs (M^3 double2) that beta is formed from.
 void compressed_beta_dims(int N, long* R, long* Rh);
-// k3 modes per wave of the on-the-fly plain kernel; its grid is
-// M * M * ceil(M / kOtfK) waves per l-slab.
+// Modes per wave of the on-the-fly plain kernel: kOtfK consecutive k3 of one
+// k-row up to M = 27, from M = 29 on groups of kOtfK modes from one orbit of
+// the cube's symmetry group (sign changes and axis permutations of k), which
+// read the same rows of F.  build_orbit_groups fills whole groups with each
+// orbit's modes, the last one padded with -1, and returns the group count
+// (`out` holds kOtfK entries per group).
 constexpr int kOtfK = 8;
+inline bool otf_orbit_layout(int N) { return 2 * ((N - 1) / 2) + 1 >= 29; }
+int build_orbit_groups(int N, std::vector<int>* out);
 int launch_genb_compressed(const BetaTables& t, double* d_T, double* d_SB, hipStream_t s);
 int launch_calq_plain_otf(int N, double rval, const double* d_T, const double* d_SB, const double* d_dup,
-                          const double* d_fc, int fc_stride, double* d_part, int nslab, hipStream_t s);
+                          const double* d_fc, int fc_stride, const int* d_kgrp, int ngroups, double* d_part,
+                          int nslab, hipStream_t s);
 
 // plain FP64 Q (calq_plain.hip)
 int launch_build_dup(int N, const double* d_fc, int fc_stride, double* d_dup, hipStream_t s);

@@ -4,7 +4,9 @@ Bmm) pairs, in place of the M^6 array of QCalculator::genB
 (QCalculator.h:158-229).
 
 * where both fit, the result equals the stored-beta FP64 kernel to FP64
-  rounding (every beta is the same double; the sums run in another order);
+  rounding (every beta is the same double; the sums run in another order),
+  with the modes of a wave taken as consecutive k3 (up to N = 28) or from one
+  symmetry orbit (from N = 29);
 * at N = 48 and 64, where the FP64 M^6 array needs 86 GB and 500 GB, selected
   modes are checked against the oracle's per-mode sum (beta from the
   reference's per-entry formula, no M^6 array) to 1e-12 of the summed |terms|,
@@ -19,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("N,alpha,fil", [(9, 0, 0), (9, 1, 1), (10, 0, 1), (16, 1, 0), (24, 0, 0),
-                                         (25, 1, 1)])
+                                         (25, 1, 1), (29, 1, 1), (32, 0, 0)])
 def test_compressed_matches_stored(edb, N, alpha, fil):
     q = edb.QCalculator(N, alpha, 8.0)
     if not fil:
