@@ -317,8 +317,9 @@ int build_orbit_groups(int N, std::vector<int>* out) {
   Geometry g(N);
   struct Mode {
     long key;  // |k|^2, then |k1|, |k2|, |k3| in decreasing order: one key per orbit
-    int kc;
+    int kc;    // (each |k_i| <= n <= 31 takes 6 bits: handles exist for N <= 64 only)
   };
+  if (g.n >= 64) return 0;
   std::vector<Mode> modes;
   modes.reserve(g.NQ);
   for (int c = 0; c < g.NQ; ++c) {

@@ -325,6 +325,7 @@ int edb_qcalc_genB_compressed(edb_qcalc* q) {
         compressed_beta_dims(q->N, &R, &Rh);
         std::vector<int> grp;
         const int ngroups = build_orbit_groups(q->N, &grp);
+        EDB_REQUIRE(ngroups >= 1, "genB_compressed: no mode groups for this grid");
         EDB_TRY(dmalloc(&q->d_ftab, (size_t)(R * Rh)));
         EDB_TRY(dmalloc(&q->d_sb, (size_t)q->g.NQ * 2));
         EDB_TRY(dmalloc(&q->d_kgrp, grp.size()));
