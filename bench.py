@@ -629,6 +629,8 @@ def main():
                      "traffic": traffic, "kernel": "calq_batched_kernel",
                      "kernel_ms_mean": round(qms, 4), "launches": nlaunch,
                      "flops_per_launch": flops_launch,
+                     "bound_note": "FP32 vector ALU (no MFMA: the weighted convolution is not GEMM-shaped); "
+                                   "peak = MI355X's FP32 157.3 TFLOP/s",
                      "flops_note": "canonical 10*M^6 per Q-eval (M = 23 modes per dim at N = 24); "
                                    "the kernel executes ~1/4 of the terms (Hermitian half x "
                                    "(l,m) pairing), so frac can exceed 1; executed_frac = the "
