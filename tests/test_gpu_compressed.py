@@ -33,6 +33,8 @@ def test_compressed_matches_stored(edb, N, alpha, fil):
         qs = q.calQ(f)
         qc = q.calQ_compressed(f)
         assert np.abs(qc - qs).max() <= 1e-12 * np.abs(qs).max()
+        # fixed summation order: a second evaluation is bitwise identical
+        np.testing.assert_array_equal(q.calQ_compressed(f), qc)
         # modes without a term (the Nyquist planes of even N, k = 0) are exactly 0 in both
         np.testing.assert_array_equal(qc == 0, qs == 0)
     if N >= 16:
