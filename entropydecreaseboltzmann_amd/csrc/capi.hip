@@ -255,8 +255,9 @@ int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out) {
   // enough l-slabs to give every SIMD several waves of the HBM stream
   q->nslab = std::max(1, std::min(64, (cu_count() * 12 + q->ntile - 1) / q->ntile));
   q->nslab = (int)std::min<long>(q->nslab, q->g.NQ);
-  // the FP32-beta stream is half as long and wants ~4x the waves in flight (measured at N = 24)
-  q->nslab32 = (int)std::max(1L, std::min<long>({4096L, q->g.NQ, (cu_count() * 48L + q->ntile - 1) / q->ntile}));
+  // the FP32-beta stream is half as long and wants ~4x the waves in flight (measured at N = 24),
+  // capped at 64 slabs like the FP64 stream so small grids do not drown in partial sums
+  q->nslab32 = (int)std::max(1L, std::min<long>({64L, q->g.NQ, (cu_count() * 48L + q->ntile - 1) / q->ntile}));
   // measured default: workgroup quads for the k3-split grids (N >= 27), where
   // they beat stream-K rows by ~10%; rows elsewhere (the autotuner times both)
   q->schedule = batched_ksplit(N) > 1 ? EDB_SCHED_QUADS : EDB_SCHED_ROWS;
@@ -733,7 +734,7 @@ int edb_qcalc_autotune(edb_qcalc* q, int batch, int* batched_waves, int* plain_s
     double best = 1e30;
     int bs = 0;
     for (int waves_per_cu : {16, 32, 48, 64, 96}) {
-      q->nslab32 = std::max(1, (int)std::min<long>({4096L, q->g.NQ, (cu * waves_per_cu + q->ntile - 1) / q->ntile}));
+      q->nslab32 = std::max(1, (int)std::min<long>({4096L, q->g.NQ / 16, (cu * waves_per_cu + q->ntile - 1) / q->ntile}));
       double ms = 0.;
       EDB_TRY(time_plain_src(q, 3, &ms, kPlainFP32));
       if (ms < best) {
