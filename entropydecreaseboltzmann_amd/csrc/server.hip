@@ -306,6 +306,13 @@ int edb_server_submit(edb_server* s, long id, int N, int alpha, double t_end, co
   EDB_REQUIRE(N >= 3 && N <= kMaxNCalq && batched_supported(N), "server_submit: unsupported N");
   EDB_REQUIRE(alpha == 0 || alpha == 1, "server_submit: alpha must be 0 or 1");
   EDB_REQUIRE(t_end > 0. && std::isfinite(t_end), "server_submit: t_end must be positive");
+  // a non-finite sample would make the whole solve NaN: refuse it here, before
+  // any device work (instances never mix, so it could not spoil the others)
+  for (long i = 0, N3 = (long)N * N * N; i < N3; ++i)
+    if (!std::isfinite(F0[i])) {
+      set_error("server_submit: initial data is not finite at sample " + std::to_string(i));
+      return EDB_EINVAL;
+    }
   if (s->p.max_pending > 0) {
     long waiting = 0;
     for (auto& kv : s->groups) waiting += (long)kv.second->pending.size();

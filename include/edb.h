@@ -314,7 +314,8 @@ typedef struct edb_server edb_server;
 int edb_server_create(const edb_server_params* p, edb_server** out);
 /* Queue one solve of f(t=0) = F0 (N^3 real on x_i = -L + i*2L/N) to t_end
  * (the drivers' step count, do { t += dt } while (t < t_end)).  EDB_ENOMEM:
- * its group can never fit the budget; EDB_EBUSY: not now (groups in use). */
+ * its group can never fit the budget; EDB_EBUSY: not now (groups in use);
+ * EDB_EINVAL: bad arguments, including a non-finite value in F0. */
 int edb_server_submit(edb_server* s, long id, int N, int alpha, double t_end, const double* F0);
 /* Advance the scheduler without blocking: retire finished chunks, admit
  * waiting requests, launch the next chunk of each group.  *ready = number of

@@ -59,6 +59,24 @@ def test_invalid_arguments_rejected_before_device_use():
     assert st.average == 1.0 and st.average_flag == 0  # EntropyFix.h:20-22
 
 
+def test_server_refuses_non_finite_initial_data():
+    """A request whose initial data holds a NaN or an infinity is refused at
+    submit with EDB_EINVAL, before any device work (a server with no groups
+    holds no device memory)."""
+    import numpy as np
+    import entropydecreaseboltzmann_amd as edb
+    srv = edb.Server(L=8.0, dt=0.01)
+    for bad in (np.nan, np.inf, -np.inf):
+        F0 = np.full((9, 9, 9), 1e-3)
+        F0[4, 2, 7] = bad
+        with pytest.raises(edb.EdbError) as e:
+            srv.submit(1, 9, 0, 1.0, F0)
+        assert e.value.status == _lib.EDB_EINVAL
+        assert b"not finite" in _lib.load().edb_last_error()
+    with pytest.raises(edb.EdbError):
+        srv.submit(1, 9, 0, float("nan"), np.full((9, 9, 9), 1e-3))  # t_end
+
+
 def test_fft_rejects_bad_lengths():
     lib = _lib.load()
     buf = (ctypes.c_double * 2)()

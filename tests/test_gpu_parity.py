@@ -336,6 +336,28 @@ def test_instance_same_alone_and_in_mix(edb):
     assert rel(alone.get_spectral()[0], mix.get_spectral()[37]) < 1e-12
 
 
+def test_non_finite_instance_stays_alone(edb):
+    """A NaN in one instance's initial data stays in that instance: the other 69
+    come out bitwise identical to the same batch with that instance finite (one
+    lane per instance in the Q kernel, per-instance transforms, entropy and fix),
+    and the solve still completes."""
+    N, L, dt = 9, 8.0, 0.02
+    F0 = _initial(N, L, 70, seed=31)
+    out = []
+    for poison in (False, True):
+        F = F0.copy()
+        if poison:
+            F[5, 4, 4, 4] = np.nan
+        s = edb.Solver(N, 0, L, False, 70, edb.PREC_FP32_BATCHED, "RK4", dt, 0.0, edb.FIX_APPLIED)
+        s.set_initial_real(F)
+        s.step(3)
+        out.append(s.get_spectral())
+    keep = np.arange(70) != 5
+    assert np.isfinite(out[0]).all()
+    assert np.array_equal(out[0][keep], out[1][keep])
+    assert not np.isfinite(out[1][5]).all()
+
+
 def test_driver_real_and_means_with_diagnostic_fix(edb):
     """In the diagnostic mode the driver's real-space F holds the fixed values
     (the state does not): F_fix = (1 - b) F + b * avg with the EntropyFix
