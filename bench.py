@@ -46,23 +46,31 @@ def mixture(N, L, seed):
     return f / (f.sum() * h ** 3)
 
 
-def cpu_baseline(N, L, budget_s=12.0):
-    """FP64 CPU oracle Q-evaluations at N (OpenMP over k) for ~budget_s."""
+def cpu_baseline(N, L, budget_s=12.0, single_budget_s=5.0):
+    """FP64 CPU oracle Q-evaluations at N (OpenMP over k) for ~budget_s, then
+    the same on one thread for ~single_budget_s."""
     import oracle
     threads = min(16, os.cpu_count() or 1)
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     B = oracle.genB(N, 0, L, False)
     f = np.fft.fftn(mixture(N, L, 12345))
-    n, t0 = 0, time.perf_counter()
-    while True:
-        oracle.calQ(N, B, f, nthreads=threads)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 200:
-            break
+
+    def timed(nthreads, budget, cap):
+        n, t0 = 0, time.perf_counter()
+        while True:
+            oracle.calQ(N, B, f, nthreads=nthreads)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget or n >= cap:
+                return n, el
+
+    n, el = timed(threads, budget_s, 200)
+    n1, el1 = timed(1, single_budget_s, 50)
     return {"value": n / el, "unit": "Q-evals/s", "cores": threads, "kind": "port",
             "sample": f"{n} FP64 Q-evaluations of one N={N} instance (oracle calQ, OpenMP over k, "
-                      f"{threads} threads) in {el:.1f} s; genB excluded"}
+                      f"{threads} threads) in {el:.1f} s; genB excluded",
+            "single_core_value": n1 / el1,
+            "single_core_sample": f"{n1} of the same Q-evaluations on one thread in {el1:.1f} s"}
 
 
 def pair_terms(N):
