@@ -45,6 +45,9 @@ int ensure_tables(edb_qcalc* q) {
 }
 
 int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out) {
+  // solvers sharing q may ask from several threads; entries are never removed
+  // before destroy, so the pointer stays valid after the lock is released
+  std::lock_guard<std::mutex> lk(q->prefix_mu);
   auto key = std::make_pair(groups, hermitian + 2 * q->schedule);
   auto it = q->prefixes.find(key);
   if (it == q->prefixes.end()) {
@@ -71,15 +74,25 @@ int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out) {
   return kOk;
 }
 
-int ensure_slots(edb_qcalc* q, int nwaves) {
-  size_t need = batched_slot_doubles(q->N, nwaves) + (size_t)(nwaves + 1) / 2;
-  if (need <= q->slots_count) return kOk;
-  if (q->d_slots) q->retired.push_back(q->d_slots);
-  q->d_slots = nullptr;
-  q->slots_count = 0;
-  EDB_TRY(dmalloc(&q->d_slots, need));
-  q->slots_count = need;
+int ensure_slots(Workspace* ws, int N, int nwaves) {
+  size_t need = batched_slot_doubles(N, nwaves) + (size_t)(nwaves + 1) / 2;
+  if (need <= ws->slots_count) return kOk;
+  if (ws->d_slots) ws->retired.push_back(ws->d_slots);
+  ws->d_slots = nullptr;
+  ws->slots_count = 0;
+  EDB_TRY(dmalloc(&ws->d_slots, need));
+  ws->slots_count = need;
   return kOk;
+}
+
+void ws_free(Workspace* w) {
+  dfree(&w->d_slots);
+  dfree(&w->d_dup);
+  dfree(&w->d_part);
+  for (void* p : w->retired) (void)hipFree(p);
+  w->retired.clear();
+  w->slots_count = 0;
+  w->part_slabs = 0;
 }
 
 int ensure_moment_table(edb_qcalc* q) {
@@ -99,38 +112,36 @@ int ensure_moment_table(edb_qcalc* q) {
   return kOk;
 }
 
-int batched_shape(edb_qcalc* q, int groups, int hermitian, PrefixEntry** pe, int* count) {
+int batched_count(const edb_qcalc* q) {
+  if (q->schedule == 1)
+    return q->nblocks_quad > 0 ? q->nblocks_quad : cu_count() * 16 / quad_waves_per_block(q->N);
+  return q->nwaves > 0 ? q->nwaves : cu_count() * 4 * batched_waves_per_simd(q->N);
+}
+
+int batched_shape(edb_qcalc* q, Workspace* ws, int groups, int hermitian, PrefixEntry** pe, int* count) {
   EDB_TRY(get_prefix(q, groups, hermitian ? 1 : 0, pe));
-  if (q->schedule == 1) {
-    const int wpb = quad_waves_per_block(q->N);
-    int nb = q->nblocks_quad > 0 ? q->nblocks_quad : cu_count() * 16 / wpb;
-    if (nb > (*pe)->total) nb = (int)(*pe)->total;
-    EDB_TRY(ensure_slots(q, nb * wpb));
-    *count = nb;
-  } else {
-    int nwaves = q->nwaves > 0 ? q->nwaves : cu_count() * 4 * batched_waves_per_simd(q->N);
-    if (nwaves > (*pe)->total) nwaves = (int)(*pe)->total;
-    EDB_TRY(ensure_slots(q, nwaves));
-    *count = nwaves;
-  }
+  int c = batched_count(q);
+  if (c > (*pe)->total) c = (int)(*pe)->total;
+  EDB_TRY(ensure_slots(ws, q->N, q->schedule == 1 ? c * quad_waves_per_block(q->N) : c));
+  *count = c;
   return kOk;
 }
 
 // Batched Q on device arrays already in the kernel layout.
-int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, const float* d_fin,
-                   hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+int batched_device(edb_qcalc* q, Workspace* ws, int batch, int ld, int hermitian, double* d_q,
+                   const float* d_fin, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   EDB_REQUIRE(q->d_pair, "calQ_batched: genB_batched has not run");
   EDB_REQUIRE(batch >= 1 && ld % 64 == 0 && batch <= ld, "calQ_batched: bad batch / ld");
   const int groups = (batch + 63) / 64;
   PrefixEntry* pe = nullptr;
   int count = 0;
-  EDB_TRY(batched_shape(q, groups, hermitian, &pe, &count));
+  EDB_TRY(batched_shape(q, ws, groups, hermitian, &pe, &count));
   if (q->schedule == 1)
     return launch_calq_quad_impl(q->N, q->d_pair, q->nblocks, q->d_sched, q->d_rowinfo, q->d_quadinfo, q->d_rowidx,
                                  q->d_ublk, pe->d_prefix, pe->nitems, pe->total, d_fin, ld, groups,
-                                 hermitian ? 1 : 0, d_q, q->d_slots, count, s, ev0, ev1);
+                                 hermitian ? 1 : 0, d_q, ws->d_slots, count, s, ev0, ev1);
   return launch_calq_batched_impl(q->N, q->d_pair, q->nblocks, q->d_sched, q->d_rowinfo, pe->d_prefix, pe->nitems,
-                                  pe->total, d_fin, ld, groups, hermitian ? 1 : 0, d_q, q->d_slots,
+                                  pe->total, d_fin, ld, groups, hermitian ? 1 : 0, d_q, ws->d_slots,
                                   count, s, ev0, ev1);
 }
 
@@ -160,41 +171,41 @@ int plain_slabs(const edb_qcalc* q, int src) {
 // The difference table and room for `slabs` slab partial sums.  A partial
 // buffer that is too small is retired (kernels in flight may still read it)
 // and replaced.
-int ensure_plain_scratch(edb_qcalc* q, int slabs) {
-  if (!q->d_dup) {
+int ensure_plain_scratch(const edb_qcalc* q, Workspace* ws, int slabs) {
+  if (!ws->d_dup) {
     long W = 2 * q->g.M - 1;
-    EDB_TRY(dmalloc(&q->d_dup, (size_t)(W * W * W * 2)));
+    EDB_TRY(dmalloc(&ws->d_dup, (size_t)(W * W * W * 2)));
   }
-  if (q->part_slabs < slabs) {
-    if (q->d_part) q->retired.push_back(q->d_part);
-    q->d_part = nullptr;
-    q->part_slabs = 0;
-    EDB_TRY(dmalloc(&q->d_part, (size_t)slabs * q->g.NQ * 2));
-    q->part_slabs = slabs;
+  if (ws->part_slabs < slabs) {
+    if (ws->d_part) ws->retired.push_back(ws->d_part);
+    ws->d_part = nullptr;
+    ws->part_slabs = 0;
+    EDB_TRY(dmalloc(&ws->d_part, (size_t)slabs * q->g.NQ * 2));
+    ws->part_slabs = slabs;
   }
   return kOk;
 }
 
 // Plain FP64 Q of one column: d_fc FP64 centred [M^3] with stride fc_stride
 // (complex elements), result centred into d_q with stride q_stride.
-int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, int q_stride,
+int plain_column(edb_qcalc* q, Workspace* ws, const double* d_fc, int fc_stride, double* d_q, int q_stride,
                  hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, int src) {
   if (src == kPlainFP32) EDB_REQUIRE(q->d_plain32, "calQ_plain32: genB_plain32 has not run");
   else if (src == kPlainCompressed) EDB_REQUIRE(q->d_ftab, "calQ_compressed: genB_compressed has not run");
   else EDB_REQUIRE(q->d_plain, "calQ: genB has not run");
   const int ns = plain_slabs(q, src);
-  EDB_TRY(ensure_plain_scratch(q, ns));
-  EDB_TRY(launch_build_dup(q->N, d_fc, fc_stride, q->d_dup, s));
+  EDB_TRY(ensure_plain_scratch(q, ws, ns));
+  EDB_TRY(launch_build_dup(q->N, d_fc, fc_stride, ws->d_dup, s));
   if (ev0) EDB_HIP_TRY(hipEventRecord(ev0, s));
   if (src == kPlainFP32)
-    EDB_TRY(launch_calq_plain32(q->N, q->d_plain32, q->d_dup, d_fc, fc_stride, q->d_part, ns, s));
+    EDB_TRY(launch_calq_plain32(q->N, q->d_plain32, ws->d_dup, d_fc, fc_stride, ws->d_part, ns, s));
   else if (src == kPlainCompressed)
-    EDB_TRY(launch_calq_plain_otf(q->N, q->tables.rval, q->d_ftab, q->d_sb, q->d_dup, d_fc, fc_stride, q->d_kgrp,
-                                  q->n_kgrp, q->d_part, ns, s));
+    EDB_TRY(launch_calq_plain_otf(q->N, q->tables.rval, q->d_ftab, q->d_sb, ws->d_dup, d_fc, fc_stride,
+                                  q->d_kgrp, q->n_kgrp, ws->d_part, ns, s));
   else
-    EDB_TRY(launch_calq_plain64(q->N, q->d_plain, q->d_dup, d_fc, fc_stride, q->d_part, ns, s));
+    EDB_TRY(launch_calq_plain64(q->N, q->d_plain, ws->d_dup, d_fc, fc_stride, ws->d_part, ns, s));
   if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
-  EDB_TRY(launch_reduce_plain(q->N, q->d_part, ns, d_q, q_stride, s));
+  EDB_TRY(launch_reduce_plain(q->N, ws->d_part, ns, d_q, q_stride, s));
   return kOk;
 }
 
@@ -416,7 +427,7 @@ int calq_device_impl(edb_qcalc* q, double* d_qout, const double* d_fin, void* st
   EDB_TRY(dmalloc(&fc, (size_t)q->g.NQ * 2));
   int st = dmalloc(&qc, (size_t)q->g.NQ * 2);
   if (st == kOk) st = launch_pack(q->N, 1, 1, false, fc, d_fin, nullptr, 0., s);
-  if (st == kOk) st = plain_column(q, fc, 1, qc, 1, s, nullptr, nullptr, src);
+  if (st == kOk) st = plain_column(q, &q->ws, fc, 1, qc, 1, s, nullptr, nullptr, src);
   if (st == kOk && q->conserve) st = launch_moment_correct(q->d_mom, q->mom_E, 1, 1, qc, s);
   if (st == kOk) st = launch_unpack(q->N, 1, 1, d_qout, qc, s);
   if (st == kOk && hipStreamSynchronize(s) != hipSuccess) {
@@ -468,7 +479,7 @@ int edb_qcalc_calQ_batched_device(edb_qcalc* q, int batch, int ld, int hermitian
                                   const float* d_fin, void* stream) {
   EDB_REQUIRE(q && d_qout && d_fin, "calQ_batched_device: null argument");
   hipStream_t s = stream ? (hipStream_t)stream : q->stream;
-  EDB_TRY(batched_device(q, batch, ld, hermitian, d_qout, d_fin, s));
+  EDB_TRY(batched_device(q, &q->ws, batch, ld, hermitian, d_qout, d_fin, s));
   if (q->conserve) EDB_TRY(launch_moment_correct(q->d_mom, q->mom_E, batch, ld, d_qout, s));
   return EDB_OK;
 }
@@ -491,7 +502,7 @@ int edb_qcalc_calQ_batched(edb_qcalc* q, int batch, int hermitian, double* qout,
   }
   if (st == kOk) st = launch_gather_cols(N3, batch, ld, dhost, dgrid, s);
   if (st == kOk) st = launch_pack(q->N, batch, ld, true, din, dgrid, nullptr, 0., s);
-  if (st == kOk) st = batched_device(q, batch, ld, hermitian, dq, din, s);
+  if (st == kOk) st = batched_device(q, &q->ws, batch, ld, hermitian, dq, din, s);
   if (st == kOk && q->conserve) st = launch_moment_correct(q->d_mom, q->mom_E, batch, ld, dq, s);
   if (st == kOk) st = launch_unpack(q->N, batch, ld, dgrid, dq, s);
   if (st == kOk) st = launch_scatter_cols(N3, batch, ld, dgrid, dhost, s);
@@ -584,7 +595,7 @@ static int batched_events_ms(edb_qcalc* q, BatchedTiming* T, int batch, int herm
   int st = kOk;
   if (hipEventRecord(T->e0, q->stream) != hipSuccess) st = kHipError;
   for (int r = 0; st == kOk && r < launches; ++r)
-    st = batched_device(q, batch, T->ld, hermitian, T->dq, T->din, q->stream);
+    st = batched_device(q, &q->ws, batch, T->ld, hermitian, T->dq, T->din, q->stream);
   if (st == kOk && hipEventRecord(T->e1, q->stream) != hipSuccess) st = kHipError;
   if (st == kOk && hipEventSynchronize(T->e1) != hipSuccess) st = kHipError;
   float el = 0.f;
@@ -605,7 +616,8 @@ static int batched_steady_ms(edb_qcalc* q, BatchedTiming* T, int batch, int herm
   EDB_TRY(batched_events_ms(q, T, batch, hermitian, 1, &est));  // first call: plans, prefix tables
   EDB_TRY(batched_events_ms(q, T, batch, hermitian, 1, &est));
   const int warm = (int)std::min(400.0, std::max(2.0, std::ceil(40.0 / std::max(est, 1e-3))));
-  for (int r = 0; r < warm; ++r) EDB_TRY(batched_device(q, batch, T->ld, hermitian, T->dq, T->din, q->stream));
+  for (int r = 0; r < warm; ++r)
+    EDB_TRY(batched_device(q, &q->ws, batch, T->ld, hermitian, T->dq, T->din, q->stream));
   return batched_events_ms(q, T, batch, hermitian, reps, ms);
 }
 
@@ -630,9 +642,10 @@ static int time_plain_src(edb_qcalc* q, int reps, double* ms, int src) {
       st = kHipError;
   }
   if (st == kOk && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = kHipError;
-  if (st == kOk) st = plain_column(q, fc, 1, dq, 1, q->stream, nullptr, nullptr, src);
+  if (st == kOk) st = plain_column(q, &q->ws, fc, 1, dq, 1, q->stream, nullptr, nullptr, src);
   if (st == kOk && hipEventRecord(e0, q->stream) != hipSuccess) st = kHipError;
-  for (int r = 0; st == kOk && r < reps; ++r) st = plain_column(q, fc, 1, dq, 1, q->stream, nullptr, nullptr, src);
+  for (int r = 0; st == kOk && r < reps; ++r)
+    st = plain_column(q, &q->ws, fc, 1, dq, 1, q->stream, nullptr, nullptr, src);
   if (st == kOk && hipEventRecord(e1, q->stream) != hipSuccess) st = kHipError;
   if (st == kOk && hipEventSynchronize(e1) != hipSuccess) st = kHipError;
   float el = 0.f;
@@ -767,17 +780,14 @@ int edb_qcalc_destroy(edb_qcalc* q) {
   dfree(&q->d_sb);
   dfree(&q->d_kgrp);
   dfree(&q->d_plain);
-  dfree(&q->d_dup);
-  dfree(&q->d_part);
+  ws_free(&q->ws);
   dfree(&q->d_pair);
   dfree(&q->d_rowinfo);
   dfree(&q->d_sched);
   dfree(&q->d_quadinfo);
   dfree(&q->d_rowidx);
   dfree(&q->d_ublk);
-  dfree(&q->d_slots);
   for (auto& kv : q->prefixes) dfree(&kv.second.d_prefix);
-  for (void* p : q->retired) (void)hipFree(p);
   if (q->tables_ready) beta_tables_free(&q->tables);
   if (q->stream) (void)hipStreamDestroy(q->stream);
   delete q;
@@ -892,6 +902,9 @@ struct edb_solver {
   edb_solver_params p{};
   edb_qcalc* qc = nullptr;
   bool own_qc = false;
+  // Q scratch of this solver's stream (not the qcalc's, which other solvers
+  // sharing the collision kernel may be using at the same time)
+  edb::Workspace ws;
   Geometry g;
   int ld = 0;
   hipStream_t stream = nullptr;
@@ -949,12 +962,13 @@ int qeval(edb_solver* s, const double* X, const double* Qprev, double c0, double
   if (batched(s)) {
     hipEvent_t e0, e1;
     EDB_TRY(next_events(s, &e0, &e1));
-    EDB_TRY(batched_device(s->qc, batch, ld, s->p.hermitian, Qout, (const float*)s->in, s->stream, e0, e1));
+    EDB_TRY(batched_device(s->qc, &s->ws, batch, ld, s->p.hermitian, Qout, (const float*)s->in, s->stream, e0,
+                           e1));
   } else {
     for (int i = 0; i < batch; ++i) {
       hipEvent_t e0, e1;
       EDB_TRY(next_events(s, &e0, &e1));
-      EDB_TRY(plain_column(s->qc, (const double*)s->in + 2 * i, ld, Qout + 2 * i, ld, s->stream, e0, e1,
+      EDB_TRY(plain_column(s->qc, &s->ws, (const double*)s->in + 2 * i, ld, Qout + 2 * i, ld, s->stream, e0, e1,
                            plain_source(s)));
     }
   }
@@ -1116,9 +1130,9 @@ int prepare_buffers(edb_solver* s) {
     const int groups = (s->p.batch + 63) / 64;
     PrefixEntry* pe = nullptr;
     int count = 0;
-    EDB_TRY(batched_shape(s->qc, groups, s->p.hermitian ? 1 : 0, &pe, &count));
+    EDB_TRY(batched_shape(s->qc, &s->ws, groups, s->p.hermitian ? 1 : 0, &pe, &count));
   } else {
-    EDB_TRY(ensure_plain_scratch(s->qc, plain_slabs(s->qc, plain_source(s))));
+    EDB_TRY(ensure_plain_scratch(s->qc, &s->ws, plain_slabs(s->qc, plain_source(s))));
   }
   return kOk;
 }
@@ -1485,6 +1499,7 @@ int edb_solver_destroy(edb_solver* s) {
   dfree(&s->hist_mean_fix);
   dfree((double**)&s->d_rec);
   dfree(&s->d_out);
+  ws_free(&s->ws);
   if (s->own_qc) edb_qcalc_destroy(s->qc);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;

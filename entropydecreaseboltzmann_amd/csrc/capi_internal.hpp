@@ -42,6 +42,24 @@ struct PrefixEntry {
   long total = 0;
 };
 
+// Per-evaluation scratch of the Q kernels: the batched kernel's stream-K
+// partial slots (and per-wave start items), the plain kernel's difference
+// table and l-slab partial sums.  Kernels write it on every evaluation, so
+// two streams that evaluate Q at the same time need a workspace each: a
+// QCalculator handle keeps one for its own calls and every solver owns one,
+// also when it shares the handle's collision kernel.  Buffers that grow are
+// retired, not freed, because a HIP graph captured earlier may still name
+// them; ws_free releases everything once no graph or launch uses them.
+struct Workspace {
+  double* d_slots = nullptr;
+  size_t slots_count = 0;  // doubles
+  double* d_dup = nullptr;
+  double* d_part = nullptr;
+  long part_slabs = 0;  // capacity of d_part, in slabs of M^3 complex
+  std::vector<void*> retired;
+};
+void ws_free(Workspace* w);
+
 }  // namespace edb
 
 struct edb_qcalc {
@@ -62,9 +80,8 @@ struct edb_qcalc {
   int ntile = 0;
   int nslab = 16;    // l-slabs of the FP64 plain kernel
   int nslab32 = 0;   // l-slabs of the FP32-beta plain kernel (0: nslab)
-  long part_slabs = 0;  // capacity of d_part, in slabs of M^3 complex
-  double* d_dup = nullptr;
-  double* d_part = nullptr;
+  // scratch of the handle's own evaluations (calQ*, time_*, autotune)
+  edb::Workspace ws;
   // paired FP32
   float* d_pair = nullptr;
   int4* d_rowinfo = nullptr;
@@ -73,8 +90,7 @@ struct edb_qcalc {
   int nrows_half = 0;
   long nblocks = 0;
   std::map<std::pair<int, int>, edb::PrefixEntry> prefixes;  // (groups, hermitian + 2 * schedule)
-  double* d_slots = nullptr;
-  size_t slots_count = 0;
+  std::mutex prefix_mu;
   int nwaves = 0;
   // quad schedule (schedule 1): workgroups of 4 neighbouring k-rows
   int schedule = 0;
@@ -84,9 +100,6 @@ struct edb_qcalc {
   int4* d_quadinfo = nullptr;
   int* d_rowidx = nullptr;
   int* d_ublk = nullptr;
-  // device buffers replaced by larger ones; kept until destroy because a HIP
-  // graph captured earlier may still name them
-  std::vector<void*> retired;
   // conservative moment correction
   int conserve = 0;
   double* d_mom = nullptr;
@@ -98,26 +111,29 @@ namespace edb {
 
 int ensure_tables(edb_qcalc* q);
 int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out);
-int ensure_slots(edb_qcalc* q, int nwaves);
+int ensure_slots(Workspace* ws, int N, int nwaves);
+// Persistent waves (rows) or workgroups (quads) of the batched kernel for
+// `groups` instance groups under the handle's schedule, before any cap by the
+// schedule's item count.
+int batched_count(const edb_qcalc* q);
 // Launch shape of the batched kernel for `groups` instance groups under the
 // handle's schedule: its prefix and persistent waves (rows) or workgroups
-// (quads), with the partial-sum slots allocated.
-int batched_shape(edb_qcalc* q, int groups, int hermitian, PrefixEntry** pe, int* count);
+// (quads), with the partial-sum slots allocated in ws.
+int batched_shape(edb_qcalc* q, Workspace* ws, int groups, int hermitian, PrefixEntry** pe, int* count);
 // Device table of the moment correction for q's (N, L), built on first use.
 int ensure_moment_table(edb_qcalc* q);
 // Batched Q on device arrays already in the kernel layout (float2 [M^3][ld] in,
 // double2 [M^3][ld] out); events, if given, bracket the main kernel.
-int batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_q, const float* d_fin,
-                   hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+int batched_device(edb_qcalc* q, Workspace* ws, int batch, int ld, int hermitian, double* d_q,
+                   const float* d_fin, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // Plain-path scratch: the difference table and room for `slabs` partial sums.
-int ensure_plain_scratch(edb_qcalc* q, int slabs);
+int ensure_plain_scratch(const edb_qcalc* q, Workspace* ws, int slabs);
 // l-slabs of the plain kernel for a PlainSource.
 int plain_slabs(const edb_qcalc* q, int src);
-// Plain FP64 Q of one column (centred FP64 input with stride fc_stride).
-// Plain Q of one column with the FP64 beta, or its FP32 copy when fp32.
-// Where the plain kernel takes beta from.
+// Where the plain kernel takes beta from, and the plain Q of one column
+// (centred FP64 input with stride fc_stride) with that beta.
 enum PlainSource { kPlainFP64 = 0, kPlainFP32 = 1, kPlainCompressed = 2 };
-int plain_column(edb_qcalc* q, const double* d_fc, int fc_stride, double* d_q, int q_stride,
+int plain_column(edb_qcalc* q, Workspace* ws, const double* d_fc, int fc_stride, double* d_q, int q_stride,
                  hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, int src = kPlainFP64);
 
 }  // namespace edb

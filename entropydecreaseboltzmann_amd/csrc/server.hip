@@ -32,10 +32,14 @@ using namespace edb;
 
 namespace {
 
+// Longest solve a request may ask for, in steps.
+constexpr long kMaxServerSteps = 100000000;
+
 struct Request {
   long id = 0;
   int N = 0, alpha = 0;
   double t_end = 0.;
+  double t_final = 0.;  // t after the drivers' last step (accumulated, may exceed t_end)
   int steps = 0;
   std::vector<double> F0;
 };
@@ -50,7 +54,7 @@ struct Result {
 struct Slot {
   long id = 0;
   int target = 0, done = 0;
-  double t_end = 0.;
+  double t_final = 0.;
 };
 
 struct Group {
@@ -150,7 +154,8 @@ int make_group(edb_server* s, int N, int alpha, Group** out) {
   if (st == kOk) {
     PrefixEntry* pe = nullptr;
     int count = 0;
-    for (int grp = 1; grp <= g->cap / 64 && st == kOk; ++grp) st = batched_shape(g->qc, grp, 1, &pe, &count);
+    for (int grp = 1; grp <= g->cap / 64 && st == kOk; ++grp)
+      st = batched_shape(g->qc, &g->qc->ws, grp, 1, &pe, &count);
   }
   if (st == kOk) st = launch_fill((double*)g->fixed, g->cap / 2, 0., g->stream);
   if (st == kOk && hipStreamSynchronize(g->stream) != hipSuccess) st = kHipError;
@@ -181,7 +186,7 @@ int admit(Group* g) {
     Slot sl;
     sl.id = r.id;
     sl.target = r.steps;
-    sl.t_end = r.t_end;
+    sl.t_final = r.t_final;
     g->slots.push_back(sl);
     g->h_flag[g->active] = 1;
     g->active++;
@@ -208,7 +213,7 @@ int rk4_step(Group* g, const edb_server_params& p) {
   const double dt = p.dt;
   auto qe = [&](const double* X, const double* Qp, double c0, double* Qo) -> int {
     EDB_TRY(launch_pack(N, b, ld, true, g->in, X, Qp, c0, g->stream));
-    EDB_TRY(batched_device(g->qc, b, ld, 1, Qo, g->in, g->stream));
+    EDB_TRY(batched_device(g->qc, &g->qc->ws, b, ld, 1, Qo, g->in, g->stream));
     if (p.conserve) EDB_TRY(launch_moment_correct(g->qc->d_mom, g->qc->mom_E, b, ld, Qo, g->stream));
     return kOk;
   };
@@ -262,7 +267,7 @@ int retire(edb_server* s, Group* g) {
     r.id = g->slots[fin[k]].id;
     r.N = g->N;
     r.steps = g->slots[fin[k]].done;
-    r.t = g->slots[fin[k]].t_end;
+    r.t = g->slots[fin[k]].t_final;
     r.F.assign(host.begin() + (long)k * N3, host.begin() + (long)(k + 1) * N3);
     s->results.push_back(std::move(r));
     s->finished++;
@@ -306,6 +311,10 @@ int edb_server_submit(edb_server* s, long id, int N, int alpha, double t_end, co
   EDB_REQUIRE(N >= 3 && N <= kMaxNCalq && batched_supported(N), "server_submit: unsupported N");
   EDB_REQUIRE(alpha == 0 || alpha == 1, "server_submit: alpha must be 0 or 1");
   EDB_REQUIRE(t_end > 0. && std::isfinite(t_end), "server_submit: t_end must be positive");
+  // the step count of do { t += dt } while (t < t_end) is found by that loop
+  // below; bound it first (the loop would stall once t_end/dt nears 2^53)
+  EDB_REQUIRE(t_end / s->p.dt <= (double)kMaxServerSteps,
+              "server_submit: t_end/dt exceeds " + std::to_string(kMaxServerSteps) + " steps");
   // a non-finite sample would make the whole solve NaN: refuse it here, before
   // any device work (instances never mix, so it could not spoil the others)
   for (long i = 0, N3 = (long)N * N * N; i < N3; ++i)
@@ -364,6 +373,7 @@ int edb_server_submit(edb_server* s, long id, int N, int alpha, double t_end, co
     ++n;
   } while (t < t_end);
   r.steps = n;
+  r.t_final = t;
   r.F0.assign(F0, F0 + g->g.N3);
   g->pending.push_back(std::move(r));
   s->submitted++;

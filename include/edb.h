@@ -315,13 +315,16 @@ int edb_server_create(const edb_server_params* p, edb_server** out);
 /* Queue one solve of f(t=0) = F0 (N^3 real on x_i = -L + i*2L/N) to t_end
  * (the drivers' step count, do { t += dt } while (t < t_end)).  EDB_ENOMEM:
  * its group can never fit the budget; EDB_EBUSY: not now (groups in use);
- * EDB_EINVAL: bad arguments, including a non-finite value in F0. */
+ * EDB_EINVAL: bad arguments, including a non-finite value in F0 and
+ * t_end / dt above 1e8 steps. */
 int edb_server_submit(edb_server* s, long id, int N, int alpha, double t_end, const double* F0);
 /* Advance the scheduler without blocking: retire finished chunks, admit
  * waiting requests, launch the next chunk of each group.  *ready = number of
  * results waiting to be fetched. */
 int edb_server_poll(edb_server* s, int* ready);
-/* Pop one finished solve: its id, N, steps, final t and real-space F. */
+/* Pop one finished solve: its id, N, steps, final t (the t accumulated by the
+ * drivers' loop, which may pass t_end by up to dt, as edb_solver_time reports)
+ * and real-space F. */
 int edb_server_fetch(edb_server* s, long* id, int* N, int* steps, double* t, double* F, long capacity);
 int edb_server_stats(const edb_server* s, long* submitted, long* finished, long* active, long* waiting,
                      long* hbm_used);

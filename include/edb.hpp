@@ -124,8 +124,11 @@ class EntropyFix {
   edb_efix_state st_{};
 };
 
-// BKW3D drivers on one instance.  Initial data: a user function of
-// (t, x, y, z) on x_i = -R + i*2R/N (BKW.h:121-138); by default the BKW
+// BKW3D drivers on one instance.  The collision operator is the drivers'
+// QCalculator(N0, 0) (BKW.h:204, 467): Jackson filter on (IS_FILTER = 1,
+// QCalculator.h:34) unless the constructor's `filter` is false.  Initial
+// data: a user function of (t, x, y, z) on x_i = -R + i*2R/N
+// (BKW.h:121-138); by default the BKW
 // profile of testfft.cpp:11-16, s = 1 - 0.4 exp(-t/6).  Entropy fix as in
 // the reference drivers: RK4 and SSP3 always run it (BKW.h:432, 499),
 // FEuler and CFE when setFixEntropy is on (:228, :294), RK2 and DUFORT never
@@ -137,7 +140,7 @@ class EntropyFix {
 template <class c_type, class r_type>
 class BKW3D {
  public:
-  BKW3D(int N0, r_type R, int alpha = 0, bool filter = false) : N0_(N0), R_(R), alpha_(alpha), filter_(filter) {}
+  BKW3D(int N0, r_type R, int alpha = 0, bool filter = true) : N0_(N0), R_(R), alpha_(alpha), filter_(filter) {}
   void setFixEntropy(bool on = true) { fix_ = on; }
   void setFixApplied(bool on = true) { applied_ = on; }
   void setInitial(std::function<r_type(r_type, r_type, r_type, r_type)> f) { init_ = f; }

@@ -26,7 +26,7 @@ static void dump(const std::string& path, const void* p, size_t bytes) {
 }
 
 // The six drivers with the reference's per-step output (BKW3D::setOutput),
-// each table written to <dir>/table_<scheme>.txt: N = 11, R = 6, BKW data at
+// each table written to <dir>/table_<scheme>.txt: N = 11, R = 6, filter off, BKW data at
 // s = 0.8 (T0 = 6 ln 2 for s(t) = 1 - 0.4 exp(-t/6)), dt = 0.05 to T0 + 0.4,
 // beta = 2 for CFE and DUFORT, fix on.
 static int driver_tables(const std::string& dir) {
@@ -34,7 +34,7 @@ static int driver_tables(const std::string& dir) {
   const char* names[] = {"FEuler", "RK2", "SSP3", "RK4", "CFE", "DUFORT"};
   for (int sc = 0; sc < 6; ++sc) {
     std::ofstream out(dir + "/table_" + names[sc] + ".txt");
-    edb::BKW3D<c_type, r_type> bkw(11, 6.0);
+    edb::BKW3D<c_type, r_type> bkw(11, 6.0, 0, /*filter=*/false);
     bkw.setFixEntropy(true);
     bkw.setOutput(&out, false);
     switch (sc) {
@@ -118,7 +118,8 @@ int main(int argc, char** argv) {
     dump(dir + "/fft_Ftest.bin", Ftest.data(), sizeof(double) * 2 * n);
   }
 
-  // CalQ/testBKW3D.cpp-like: a short RK4 run with the diagnostic fix.
+  // CalQ/testBKW3D.cpp-like: a short RK4 run with the diagnostic fix, the
+  // default-constructed driver (filter on, as the reference's drivers).
   {
     edb::BKW3D<c_type, r_type> bkw(9, 8.0);
     bkw.setFixEntropy(true);

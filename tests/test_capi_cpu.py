@@ -77,6 +77,22 @@ def test_server_refuses_non_finite_initial_data():
         srv.submit(1, 9, 0, float("nan"), np.full((9, 9, 9), 1e-3))  # t_end
 
 
+def test_server_refuses_too_many_steps():
+    """t_end / dt beyond 1e8 steps is refused at submit with EDB_EINVAL, before
+    the host loop that counts the drivers' steps (which would stall once t
+    stops changing near 2^53 * dt) and before any device work."""
+    import numpy as np
+    import entropydecreaseboltzmann_amd as edb
+    srv = edb.Server(L=8.0, dt=0.01)
+    F0 = np.full((9, 9, 9), 1e-3)
+    for t_end in (1.0e7, 1.0e300):
+        with pytest.raises(edb.EdbError) as e:
+            srv.submit(1, 9, 0, t_end, F0)
+        assert e.value.status == _lib.EDB_EINVAL
+        assert b"steps" in _lib.load().edb_last_error()
+    assert srv.stats()["submitted"] == 0
+
+
 def test_fft_rejects_bad_lengths():
     lib = _lib.load()
     buf = (ctypes.c_double * 2)()

@@ -35,7 +35,7 @@ def test_cpp_mirror_matches_oracle(tmp_path, edb):
     Ft = ld("fft_Ftest.bin", np.complex128).reshape(8, 8, 8)
     assert np.abs(Fh - np.fft.fftn(F)).max() <= 1e-12 * np.abs(Fh).max()
     assert np.abs(Ft - F).max() <= 1e-13 * np.abs(F).max()
-    # BKW3D RK4 with the diagnostic fix vs the oracle driver
+    # default-constructed BKW3D RK4 (filter on) with the diagnostic fix vs the oracle driver
     Fb = ld("bkw_F.bin").reshape(N, N, N)
     h = 16.0 / N
     x = -8.0 + h * np.arange(N)
@@ -43,7 +43,7 @@ def test_cpp_mirror_matches_oracle(tmp_path, edb):
     s = 1 - 0.4
     v2 = X ** 2 + Y ** 2 + Z ** 2
     F0 = (2 * np.pi * s) ** -1.5 * np.exp(-v2 / (2 * s)) * ((5 * s - 3) / (2 * s) + (1 - s) / (2 * s * s) * v2)
-    Bn = oracle.genB(N, 0, 8.0, False)
+    Bn = oracle.genB(N, 0, 8.0, True)  # BKW3D(N0, R) builds QCalculator(N0, 0): filter on
     fo, t, H, Hf = oracle.run(N, Bn, 8.0, "RK4", 0.0, 0.1, 0.02, np.fft.fftn(F0), fix_mode=1)
     ref_F = np.fft.ifftn(fo).real
     if not (np.isfinite(H[-1]) and Hf[-1] != H[-1]):  # no fix at the last step: F() is the transform of the state

@@ -15,7 +15,9 @@ def solve_alone(edb, N, alpha, t_end, F0):
                    hermitian=True)
     s.set_initial_real(F0[None])
     n = s.run(t_end)
-    return n, s.get_real()[0]
+    t, steps = s.time()
+    assert steps == n
+    return n, t, s.get_real()[0]
 
 
 def drain(srv, expect, max_polls=100000):
@@ -51,9 +53,10 @@ def test_server_results_equal_solo_solves(edb):
     assert st["finished"] == 30 and st["active"] == 0 and st["waiting"] == 0
     for rid in (0, 5, 13, 22, 29):
         N, alpha, t_end, F0 = reqs[rid]
-        n, Fa = solve_alone(edb, N, alpha, t_end, F0)
+        n, ta, Fa = solve_alone(edb, N, alpha, t_end, F0)
         Ns, steps, t, Fs = done[rid]
         assert Ns == N and steps == n
+        assert t == ta  # the accumulated t of the drivers' loop, not t_end
         assert np.abs(Fs - Fa).max() <= 1e-12 * np.abs(Fa).max(), rid
     # mass conservation of every served result
     for rid, (N, _a, _t, F0) in reqs.items():
@@ -90,3 +93,28 @@ def test_server_admission_limit(edb):
     assert sorted(done) == [0, 1, 2]
     assert srv.submit(10, 9, 0, 0.02, F0) == 0
     assert 10 in drain(srv, 1)
+
+
+def test_server_bench_sizes_equal_solo_solves(edb):
+    """Groups at the serving workload's larger sizes, N = 24 and 32 with
+    gamma 0 and 1, requests submitted while others are on the device: each
+    served result equals its solo solve to 1e-12, with the same steps and t."""
+    rng = np.random.default_rng(2432)
+    srv = edb.Server(L=L, dt=DT, fix_mode=edb.FIX_APPLIED, max_chunk=2)
+    reqs = {}
+    for rid in range(12):
+        N = (24, 32)[rid % 2]
+        alpha = (rid // 2) % 2
+        t_end = float(rng.uniform(0.015, 0.05))
+        F0 = maxwellian_mixture(N, L, 2400 + rid)
+        reqs[rid] = (N, alpha, t_end, F0)
+        assert srv.submit(rid, N, alpha, t_end, F0) == 0
+        srv.poll()  # arrivals interleaved with chunks of the groups already running
+    done = drain(srv, len(reqs))
+    assert srv.stats()["finished"] == 12
+    for rid in (0, 1, 2, 3, 10, 11):  # every (N, alpha), first and last arrivals
+        N, alpha, t_end, F0 = reqs[rid]
+        n, ta, Fa = solve_alone(edb, N, alpha, t_end, F0)
+        Ns, steps, t, Fs = done[rid]
+        assert Ns == N and steps == n and t == ta
+        assert np.abs(Fs - Fa).max() <= 1e-12 * np.abs(Fa).max(), rid
