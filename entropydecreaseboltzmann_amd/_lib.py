@@ -69,6 +69,8 @@ _SIGS = {
     "edb_qcalc_calQ_batched": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "edb_qcalc_calQ_batched_device": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                      _vp, _vp, _vp]),
+    "edb_qcalc_calQ_batched_device_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                             _vp, _vp, _vp]),
     "edb_qcalc_copy_B": (ctypes.c_int, [_vp, _dp]),
     "edb_qcalc_info": (ctypes.c_int, [_vp, _ip, _ip, _lp, _lp]),
     "edb_qcalc_time_batched": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),

@@ -136,6 +136,22 @@ int edb_qcalc_calQ_batched(edb_qcalc* q, int batch, int hermitian, double* qout,
 int edb_qcalc_calQ_batched_device(edb_qcalc* q, int batch, int ld, int hermitian, double* d_qout,
                                   const float* d_fin, void* stream);
 
+/* edb_qcalc_calQ_batched_device with no allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Same arguments and results (equal to
+ * rounding of the FP64 merge order, bitwise when both use the same persistent
+ * count); calls no hipMalloc or hipFree.  Its scratch (the work schedule's
+ * prefix table and the stream-K partial slots) is a 64 MiB file-scope device
+ * array, one per GPU, kept until the process ends; the persistent count is
+ * capped to what it holds, and EDB_EINVAL is returned when even one wave's
+ * slots and the prefix table (8 bytes per schedule item: at most ~9 KB per 64
+ * instances, N = 33 with the rows schedule) do not fit.
+ * Calls that overlap on one GPU (other streams, or a call still in flight on
+ * another stream) share the array and race; calls in order on one stream do
+ * not.  The first call on a device looks up the array's address there. */
+int edb_qcalc_calQ_batched_device_noalloc(edb_qcalc* q, int batch, int ld, int hermitian, double* d_qout,
+                                          const float* d_fin, void* stream);
+
 /* beta in the reference's layout B[l][k] (M^6 doubles, l and k centred,
  * QCalculator.h:199-224), computed on the device; for verification. */
 int edb_qcalc_copy_B(edb_qcalc* q, double* Bout);
