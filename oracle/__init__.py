@@ -8,6 +8,7 @@ compiled from their sources).  Only tests/, ``__graft_entry__.smoke()`` and
 (or to time the CPU baseline);
 the product (entropydecreaseboltzmann_amd) never does.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -37,47 +38,65 @@ def lib():
     if _o is None:
         if not os.path.exists(LIB):
             build()
-        o = ctypes.CDLL(LIB)
-        D, I = ctypes.c_double, ctypes.c_int
-        o.edbo_nmodes.restype = I
-        o.edbo_lambda.restype = D
-        o.edbo_funF.restype = D
-        o.edbo_funF.argtypes = [I, D, D]
-        o.edbo_filtering.restype = D
-        o.edbo_filtering.argtypes = [I, I]
-        o.edbo_rval.restype = D
-        o.edbo_rval.argtypes = [I, I, D]
-        o.edbo_wrap.restype = I
-        o.edbo_wrap.argtypes = [I, I, _ip]
-        o.edbo_beta.restype = D
-        o.edbo_beta.argtypes = [I, I, D, I, I, I, I, I, I, I]
-        o.edbo_genB.argtypes = [I, I, D, I, _dp]
-        o.edbo_calQ.argtypes = [I, _dp, _dp, _dp, I]
-        o.edbo_calQ_direct.argtypes = [I, I, D, I, _dp, _dp]
-        o.edbo_calQ_modes.argtypes = [I, I, D, I, _dp, I, ctypes.POINTER(ctypes.c_long), I, _dp]
-        o.edbo_dft3.argtypes = [I, I, _dp, _dp]
-        o.edbo_fft.argtypes = [I, _dp, _dp]
-        o.edbo_ifft.argtypes = [I, _dp, _dp]
-        P = ctypes.POINTER(Efix)
-        o.edbo_efix_init.argtypes = [P, ctypes.c_long, D]
-        o.edbo_set_average.argtypes = [P, ctypes.c_long, _dp]
-        o.edbo_entropy_cal.restype = D
-        o.edbo_entropy_cal.argtypes = [P, ctypes.c_long, _dp]
-        o.edbo_fixsol.argtypes = [P, ctypes.c_long, _dp, D]
-        o.edbo_flogf.restype = D
-        o.edbo_flogf.argtypes = [D]
-        o.edbo_cal_entropy.restype = D
-        o.edbo_cal_entropy.argtypes = [I, D, _dp]
-        o.edbo_bkw.restype = D
-        o.edbo_bkw.argtypes = [D, D, D, D]
-        o.edbo_grid_bkw.argtypes = [I, D, D, _dp]
-        o.edbo_run.restype = I
-        o.edbo_run.argtypes = [I, _dp, D, I, D, D, D, D, I, _dp, I, _dp, _dp, _dp, I]
-        o.edbo_run_ex.restype = I
-        o.edbo_run_ex.argtypes = [I, _dp, D, I, D, D, D, D, I, I, _dp, I, _dp, _dp, _dp, I]
-        o.edbo_moment_correct.argtypes = [I, D, _dp]
-        _o = o
+        _o = _bind(ctypes.CDLL(LIB))
     return _o
+
+
+@contextlib.contextmanager
+def using(path):
+    """Route every function of this module through another build of the
+    oracle's source (tests that plant a mistake in a copy of it and check that
+    the pins catch it)."""
+    global _o
+    saved = lib()
+    _o = _bind(ctypes.CDLL(path))
+    try:
+        yield
+    finally:
+        _o = saved
+
+
+def _bind(o):
+    """Declares the argument and result types of the oracle's entry points."""
+    D, I = ctypes.c_double, ctypes.c_int
+    o.edbo_nmodes.restype = I
+    o.edbo_lambda.restype = D
+    o.edbo_funF.restype = D
+    o.edbo_funF.argtypes = [I, D, D]
+    o.edbo_filtering.restype = D
+    o.edbo_filtering.argtypes = [I, I]
+    o.edbo_rval.restype = D
+    o.edbo_rval.argtypes = [I, I, D]
+    o.edbo_wrap.restype = I
+    o.edbo_wrap.argtypes = [I, I, _ip]
+    o.edbo_beta.restype = D
+    o.edbo_beta.argtypes = [I, I, D, I, I, I, I, I, I, I]
+    o.edbo_genB.argtypes = [I, I, D, I, _dp]
+    o.edbo_calQ.argtypes = [I, _dp, _dp, _dp, I]
+    o.edbo_calQ_direct.argtypes = [I, I, D, I, _dp, _dp]
+    o.edbo_calQ_modes.argtypes = [I, I, D, I, _dp, I, ctypes.POINTER(ctypes.c_long), I, _dp]
+    o.edbo_dft3.argtypes = [I, I, _dp, _dp]
+    o.edbo_fft.argtypes = [I, _dp, _dp]
+    o.edbo_ifft.argtypes = [I, _dp, _dp]
+    P = ctypes.POINTER(Efix)
+    o.edbo_efix_init.argtypes = [P, ctypes.c_long, D]
+    o.edbo_set_average.argtypes = [P, ctypes.c_long, _dp]
+    o.edbo_entropy_cal.restype = D
+    o.edbo_entropy_cal.argtypes = [P, ctypes.c_long, _dp]
+    o.edbo_fixsol.argtypes = [P, ctypes.c_long, _dp, D]
+    o.edbo_flogf.restype = D
+    o.edbo_flogf.argtypes = [D]
+    o.edbo_cal_entropy.restype = D
+    o.edbo_cal_entropy.argtypes = [I, D, _dp]
+    o.edbo_bkw.restype = D
+    o.edbo_bkw.argtypes = [D, D, D, D]
+    o.edbo_grid_bkw.argtypes = [I, D, D, _dp]
+    o.edbo_run.restype = I
+    o.edbo_run.argtypes = [I, _dp, D, I, D, D, D, D, I, _dp, I, _dp, _dp, _dp, I]
+    o.edbo_run_ex.restype = I
+    o.edbo_run_ex.argtypes = [I, _dp, D, I, D, D, D, D, I, I, _dp, I, _dp, _dp, _dp, I]
+    o.edbo_moment_correct.argtypes = [I, D, _dp]
+    return o
 
 
 def _p(a):
