@@ -9,10 +9,16 @@ RK4 with dt = 0.01, entropy fix applied.  Each step is 4 batched
 Q-evaluations (4 x 256 Q-evals) + inverse FFT, entropy and fix.
 
 Prints one JSON line.  value = Q-evals per second over the timed steps
-(inputs resident in HBM).  roofline: the batched Q kernel's canonical work
-(10 M^6 flops per Q-eval) over its mean launch time, measured with HIP
-events on its own stream during the timed steps.  cpu_baseline: the FP64 CPU
-oracle (OpenMP over k) on a bounded sample of the same Q-evaluation.
+(inputs resident in HBM).  roofline: the batched Q kernel's executed work
+(10 flops per evaluated pair-term) over its mean launch time, measured with
+HIP events on its own stream during the timed steps, against the FP32 peak;
+the reference's canonical 10 M^6 flops per Q-eval is reported beside it.
+graph_ms_per_step: the same steps replayed from HIP graphs.  plain_fp64: the
+single-instance FP64 path the speedup is quoted against.  p50_s / p99_s /
+req_per_s: a serving leg at a rate the device sustains (Poisson arrivals of
+mixed N in {16, 24, 32}, gamma in {0, 1}); the 200 req/s overload leg sits
+under "serving".  cpu_baseline: the FP64 CPU oracle (OpenMP over k) on a
+bounded sample of the same Q-evaluation.
 """
 import argparse
 import json
@@ -136,23 +142,30 @@ def profile_kernels(edb, N, batch, schedule=None, count=0):
 
 
 def serve(edb, args):
-    """Serving run: requests arrive as a Poisson process at --rate for --duration
+    """--serve: one serving run at --rate for --duration seconds."""
+    print(json.dumps(serve_run(edb, args.rate, args.duration, args.drain_max, args.max_pending, args.seed)),
+          flush=True)
+
+
+def serve_run(edb, rate, duration, drain_max, max_pending, seed, plain=None):
+    """Serving run: requests arrive as a Poisson process at `rate` for `duration`
     seconds, each drawing N from {16, 24, 32}, gamma from {0, 1}, t_end ~ U[1, 5]
     and a mixed-Maxwellian initial f from its own seed; RK4 dt = 0.01 with the
-    applied entropy fix.  Latency = completion - arrival (wall clock)."""
+    applied entropy fix.  Latency = completion - arrival (wall clock).  `plain`
+    caches the one-at-a-time FP64 Q-evaluation times per N between runs."""
     L, dt = 8.0, 0.01
-    rng = np.random.default_rng(args.seed)
+    rng = np.random.default_rng(seed)
     arrivals, t = [], 0.0
     while True:
-        t += rng.exponential(1.0 / args.rate)
-        if t > args.duration:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
             break
         arrivals.append(t)
     reqs = []
     for i, ta in enumerate(arrivals):
         reqs.append((i, ta, int(rng.choice([16, 24, 32])), int(rng.integers(0, 2)),
                      float(rng.uniform(1.0, 5.0))))
-    srv = edb.Server(L=L, dt=dt, fix_mode=edb.FIX_APPLIED, max_chunk=8, max_pending=args.max_pending)
+    srv = edb.Server(L=L, dt=dt, fix_mode=edb.FIX_APPLIED, max_chunk=8, max_pending=max_pending)
     # warm the six groups (collision kernels built, buffers allocated) before the clock starts
     warm = 0
     for N in (16, 24, 32):
@@ -173,8 +186,8 @@ def serve(edb, args):
     t0 = time.perf_counter()
     nxt, backlog = 0, []
     next_report = t0 + 20.0
-    end_arrivals = t0 + args.duration
-    deadline = end_arrivals + args.drain_max
+    end_arrivals = t0 + duration
+    deadline = end_arrivals + drain_max
     while True:
         now = time.perf_counter()
         while nxt < len(reqs) and t0 + reqs[nxt][1] <= now:
@@ -182,8 +195,8 @@ def serve(edb, args):
             nxt += 1
         still = []
         for (i, ta, N, g, te) in backlog:
-            st = srv.submit(i, N, g, te, mixture(N, L, args.seed * 100003 + i))
-            if st != 0 and args.max_pending > 0:
+            st = srv.submit(i, N, g, te, mixture(N, L, seed * 100003 + i))
+            if st != 0 and max_pending > 0:
                 rejected += 1  # admission limit: the request is turned away
             elif st != 0:
                 still.append((i, ta, N, g, te))
@@ -209,9 +222,13 @@ def serve(edb, args):
     wall = time.perf_counter() - t0
     lat = np.array(lat)
     st = srv.stats()
-    # one-at-a-time FP64: plain Q-eval time per N times this mix's Q-evals, plus per-step overhead
-    plain = {}
+    srv.close()
+    # one-at-a-time FP64: plain Q-eval time per N times this mix's Q-evals
+    if plain is None:
+        plain = {}
     for N in (16, 24, 32):
+        if N in plain:
+            continue
         q = edb.QCalculator(N, 0, L)
         q.closeFilter()
         q.genB()
@@ -220,8 +237,8 @@ def serve(edb, args):
         q.close()
     seq = sum(4 * steps_of(te) * plain[N] for (_i, _ta, N, _g, te) in reqs)
     out = {"metric": "serving: p50/p99 solve latency, throughput",
-           "rate_req_s": args.rate, "duration_s": args.duration, "requests": len(reqs),
-           "completed": int(len(lat)), "rejected": int(rejected), "max_pending": args.max_pending,
+           "rate_req_s": rate, "duration_s": duration, "requests": len(reqs),
+           "completed": int(len(lat)), "rejected": int(rejected), "max_pending": max_pending,
            "unfinished": int(len(reqs) - len(lat) - rejected),
            "wall_s": round(wall, 2),
            "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
@@ -230,14 +247,14 @@ def serve(edb, args):
            "throughput_req_s": round(len(lat) / wall, 2), "qevals_per_s": round(qevals / wall, 1),
            # completions in the second half of the arrival window: the rate the device sustains
            "throughput_steady_req_s": round(
-               sum(1 for tc in done_t.values() if t0 + args.duration / 2 <= tc <= end_arrivals)
-               / (args.duration / 2), 2),
+               sum(1 for tc in done_t.values() if t0 + duration / 2 <= tc <= end_arrivals)
+               / (duration / 2), 2),
            "one_at_a_time_fp64_req_s": round(len(reqs) / seq, 3),
            "one_at_a_time_note": "plain FP64 Q-evaluation time per N (autotuned) x 4 Q-evals per RK4 "
                                  "step of each request, summed over this request mix",
            "speedup_vs_one_at_a_time": round((len(lat) / wall) / (len(reqs) / seq), 1),
            "server": st}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 # ------------------------------------------------------------------ experiments
@@ -540,6 +557,15 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay each RK4 step from a HIP graph")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving legs of the default run")
+    ap.add_argument("--serve-rate", type=float, default=32.0,
+                    help="arrival rate of the default run's sustained serving leg (req/s)")
+    ap.add_argument("--serve-duration", type=float, default=8.0,
+                    help="seconds of arrivals of the sustained serving leg")
+    ap.add_argument("--overload-duration", type=float, default=4.0,
+                    help="seconds of 200 req/s arrivals of the overload serving leg")
+    ap.add_argument("--graph-steps", type=int, default=100,
+                    help="steps replayed from HIP graphs after the timed eager steps (graph_ms_per_step)")
     ap.add_argument("--serve", action="store_true",
                     help="serving run: Poisson arrivals of mixed (N, gamma, t_end) solves")
     ap.add_argument("--rate", type=float, default=200.0, help="requests per second (--serve)")
@@ -611,11 +637,22 @@ def main():
         qms, nlaunch = s.qkernel_time()
     qevals = 4 * batch * args.steps
     value = qevals / el
-    flops_launch = 10.0 * M ** 6 * batch
+    flops_launch = 10.0 * M ** 6 * batch  # canonical: the reference's 10 flops per (k, l) term
     executed = pair_terms(N) * 64 * ((batch + 63) // 64) * 10.0  # 4 FMA + 2 MUL per pair-term
-    achieved = flops_launch / (qms * 1e-3) / 1e12
+    achieved = executed / (qms * 1e-3) / 1e12
+    canonical = flops_launch / (qms * 1e-3) / 1e12
     traffic = traffic_from_profiles()
     mom = s.moments()
+    graph_ms = None
+    if not args.graph and args.graph_steps > 0:  # the same steps replayed from HIP graphs
+        s.use_graph(True)
+        s.enable_qtiming(False)
+        s.step(2)  # capture (first step and steady-state graphs) outside the clock
+        edb.load().edb_sync()
+        tg = time.perf_counter()
+        s.step(args.graph_steps)
+        edb.load().edb_sync()
+        graph_ms = (time.perf_counter() - tg) / args.graph_steps * 1e3
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -636,19 +673,27 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic, "kernel": "calq_batched_kernel",
                      "kernel_ms_mean": round(qms, 4), "launches": nlaunch,
-                     "flops_per_launch": flops_launch,
+                     "flops_per_launch": executed,
                      "bound_note": "FP32 vector ALU (no MFMA: the weighted convolution is not GEMM-shaped); "
                                    "peak = MI355X's FP32 157.3 TFLOP/s",
-                     "flops_note": "canonical 10*M^6 per Q-eval (M = 23 modes per dim at N = 24); "
-                                   "the kernel executes ~1/4 of the terms (Hermitian half x "
-                                   "(l,m) pairing), so frac can exceed 1; executed_frac = the "
-                                   "executed flops (4 FMA + 2 MUL per pair-term) over the same peak; "
-                                   "issue_frac = its 6 VALU instructions per pair-term over the "
-                                   "dual-issue FP32 VALU rate (1024 SIMDs x 2.4 GHz x 2 per 4 cycles)",
-                     "executed_tflops": round(executed / (qms * 1e-3) / 1e12, 2),
-                     "executed_frac": round(executed / (qms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                     "flops_note": "executed work: 4 FMA + 2 MUL = 10 flops per evaluated pair-term of "
+                                   "the Hermitian-half paired schedule (pair_terms(N) per 64 instances), "
+                                   "over the kernel's mean launch time from HIP events on the solver's "
+                                   "stream in the timed steps; issue_frac = its 6 VALU instructions per "
+                                   "pair-term over the dual-issue FP32 VALU rate (1024 SIMDs x 2.4 GHz x "
+                                   "2 per 4 cycles)",
+                     "traffic_note": "HBM bytes per launch from profiles/calq_batched_n24_b256_traffic.json "
+                                     "(rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this kernel at "
+                                     "N=24 x 256, scaled by a device copy), not measured in this run",
+                     "canonical_flops_per_launch": flops_launch,
+                     "canonical_tflops_equiv": round(canonical, 2),
+                     "canonical_note": "the reference's 10 flops per (k, l) term, 10*M^6 per Q-eval "
+                                       "(M = 23 modes per dim at N = 24), over the same time: ~4.5x the "
+                                       "executed work, because the kernel evaluates the Hermitian half "
+                                       "and sums each (l, m) pair once",
                      "issue_frac": round(executed * 0.6 / 64 / (qms * 1e-3) / (1024 * 2.4e9 / 2), 4),
                      "batched_waves": tuned["batched_waves"]},
+        "graph_ms_per_step": round(graph_ms, 4) if graph_ms else None,
         "mass_drift_max": float(np.abs(mom[:, 0] - 1.0).max()),
     }
     if not args.no_plain:
@@ -669,6 +714,26 @@ def main():
                                         "stored_beta_bytes": qp.info()["bytes_plain64"]}
         out["speedup_vs_plain_fp64_compressed"] = round(value / out["plain_fp64_compressed"]["qevals_per_s"], 2)
         qp.close()
+    s.close()
+    q.close()
+    if not args.no_serve:
+        # serving legs: a rate the device sustains (latency = solve time), then the
+        # 200 req/s arrival rate of the serving workload for a short window (overload:
+        # latency is queueing time)
+        plain_cache = {}
+        sus = serve_run(edb, args.serve_rate, args.serve_duration, 60.0, 0, args.seed, plain_cache)
+        ovl = serve_run(edb, 200.0, args.overload_duration, 90.0, 0, args.seed + 1, plain_cache)
+        out["p50_s"] = sus["p50_s"]
+        out["p99_s"] = sus["p99_s"]
+        out["req_per_s"] = sus["throughput_req_s"]
+        keep = ("rate_req_s", "duration_s", "requests", "completed", "unfinished", "wall_s", "p50_s", "p90_s",
+                "p99_s", "throughput_req_s", "throughput_steady_req_s", "qevals_per_s",
+                "one_at_a_time_fp64_req_s", "speedup_vs_one_at_a_time")
+        out["serving"] = {"workload": "Poisson arrivals, N in {16, 24, 32}, gamma in {0, 1}, t_end ~ U[1, 5], "
+                                      "mixed-Maxwellian data per request, RK4 dt = 0.01, fix applied; "
+                                      "latency = completion - arrival",
+                          "sustained": {k: sus[k] for k in keep},
+                          "overload_200_req_s": {k: ovl[k] for k in keep}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, L)
     print(json.dumps(out), flush=True)

@@ -161,6 +161,18 @@ class QCalculator:
               "autotune")
         return {"batched_waves": bw.value, "plain_slabs": ps.value}
 
+    def tune_plain(self, kernel=-1, slabs=0, fp32=False):
+        """Stored-beta stream variant (0: per-term gather of f_[k-l]; 1-3: f_[k-l]
+        from an LDS window, beta in groups of 4/2/8; -1 keeps it) and l-slab
+        count (0 keeps it) of the plain kernel."""
+        check(load().edb_qcalc_tune_plain(self._h, int(bool(fp32)), int(kernel), int(slabs)), "tune_plain")
+
+    def plain_schedule(self, fp32=False):
+        k, n = ctypes.c_int(), ctypes.c_int()
+        check(load().edb_qcalc_plain_schedule(self._h, int(bool(fp32)), ctypes.byref(k), ctypes.byref(n)),
+              "plain_schedule")
+        return k.value, n.value
+
     def tune_batched(self, schedule, count=0):
         """schedule 0: stream-K rows (count waves); 1: quads (count workgroups)."""
         check(load().edb_qcalc_tune_batched(self._h, int(schedule), int(count)), "tune_batched")

@@ -72,6 +72,8 @@ _SIGS = {
     "edb_qcalc_calQ_batched_device_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                              _vp, _vp, _vp]),
     "edb_qcalc_copy_B": (ctypes.c_int, [_vp, _dp]),
+    "edb_qcalc_tune_plain": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "edb_qcalc_plain_schedule": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
     "edb_qcalc_info": (ctypes.c_int, [_vp, _ip, _ip, _lp, _lp]),
     "edb_qcalc_time_batched": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "edb_qcalc_time_plain": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),

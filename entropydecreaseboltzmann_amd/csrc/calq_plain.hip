@@ -18,6 +18,8 @@
 // a fixed order (deterministic).
 #include <algorithm>
 #include <cstdlib>
+#include <string>
+#include <type_traits>
 
 #include "edb_internal.hpp"
 
@@ -90,6 +92,108 @@ __global__ __launch_bounds__(64) void calq_plain64_kernel(int M, long NQ, const 
     double2 a = fc[l * stride];
     accum_term(a, c, b, qr, qi);
     if (++a3 == M) { a3 = 0; if (++a2 == M) { a2 = 0; ++a1; } }
+  }
+  if (kc < NQ) part[(long)slab * NQ + kc] = make_double2(qr, qi);
+}
+
+// The same stream with the difference table read through LDS.  For one
+// l-row (l1, l2) the wave's lanes (64 consecutive centred k, spanning at most
+// rq = ceil(64/M) + 1 k-rows) read D only on rows r_q = (k1_q - l1 + M-1) W +
+// (k2_q - l2 + M-1) of their k-rows q, at position k3 - l3 + M-1: rq x W
+// entries (2.9 KB at N = 24) instead of one 16-byte vector gather per term.
+// The window of the next l-row is loaded into registers while the current
+// one is summed and stored to the other LDS buffer at the row's end, so the
+// vector-memory unit carries only the beta stream and a few window loads per
+// row, and each term reads its f_[k-l] with one ds_read_b128.  Terms are
+// summed in the same l order with the same arithmetic as
+// calq_plain64_kernel: the results are bitwise identical.
+constexpr int kWinMaxLoads = 6;  // window loads per lane and row: rq W <= 128 + 4M <= 6 * 64 for M <= 63
+
+__host__ __device__ inline int win_rows(int M) { return (64 + M - 1) / M + 1; }
+// Window loads per lane and row, rounded up to a compiled count (3, 4 or 6):
+// the staging registers are the kernel's largest VGPR item after the beta group.
+inline int win_loads(int M) {
+  const int n = (win_rows(M) * (2 * M - 1) + 63) / 64;
+  return n <= 3 ? 3 : (n <= 4 ? 4 : 6);
+}
+
+template <int U, int NST, typename T>
+__global__ __launch_bounds__(64) void calq_plain64_lds_kernel(int M, long NQ, const T* __restrict__ B,
+                                                              const double2* __restrict__ dup,
+                                                              const double2* __restrict__ fc, int stride,
+                                                              double2* __restrict__ part, int nslab) {
+  extern __shared__ double2 win[];  // [2][rq * W]
+  const int W = 2 * M - 1, rq = win_rows(M), wsz = rq * W;
+  const long MM = (long)M * M;
+  const int lane = threadIdx.x;
+  const int tile = blockIdx.x, slab = blockIdx.y;
+  const long kc = (long)tile * 64 + lane;
+  const long kcc = kc < NQ ? kc : NQ - 1;  // padded lanes read a valid address; their beta is 0
+  const int k3 = (int)(kcc % M);
+  const long kr0 = ((long)tile * 64) / M;  // first k-row (k1 M + k2) of the wave
+  const int q = (int)(kcc / M - kr0);       // this lane's k-row among the wave's, < rq
+  const long l0 = NQ * slab / nslab, l1e = NQ * (slab + 1) / nslab;
+  const long kp = (long)gridDim.x * 64;  // padded row length
+  const T* bt = B + (long)tile * 64 + lane;
+  double qr = 0., qi = 0.;
+  double2 st[NST];
+  auto window_load = [&](long lr) {  // D-rows of l-row lr (l1 = lr / M, l2 = lr % M) into registers
+    const int l1 = (int)(lr / M), l2 = (int)(lr % M);
+#pragma unroll
+    for (int i = 0; i < NST; ++i) {
+      const int e = lane + 64 * i;
+      const int qq = e / W, p = e - qq * W;
+      const long kr = kr0 + qq;
+      st[i] = make_double2(0., 0.);
+      if (e < wsz && kr < MM) {
+        const int k1q = (int)(kr / M), k2q = (int)(kr - (long)k1q * M);
+        const long drow = (long)(k1q - l1 + M - 1) * W + (k2q - l2 + M - 1);
+        st[i] = dup[drow * W + p];
+      }
+    }
+  };
+  auto window_store = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NST; ++i) {
+      const int e = lane + 64 * i;
+      if (e < wsz) win[buf * wsz + e] = st[i];
+    }
+  };
+  long lr = l0 / M;
+  const long lr_end = (l1e - 1) / M;
+  window_load(lr);
+  window_store(0);
+  __syncthreads();
+  int buf = 0;
+  for (; lr <= lr_end; ++lr) {
+    if (lr < lr_end) window_load(lr + 1);
+    const long la = lr * M > l0 ? lr * M : l0;
+    const long lb = (lr + 1) * M < l1e ? (lr + 1) * M : l1e;
+    const double2* wrow = win + buf * wsz + q * W + k3 + M - 1 - (la - lr * M);  // entry of l = la
+    long l = la;
+    for (; l + U <= lb; l += U) {
+      double b[U];
+      double2 c[U], a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        b[u] = (double)__builtin_nontemporal_load(bt + (l + u) * kp);
+        a[u] = fc[(l + u) * stride];
+        c[u] = wrow[-(int)(l + u - la)];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) accum_term(a[u], c[u], b[u], qr, qi);
+    }
+    for (; l < lb; ++l) {
+      const double b = (double)__builtin_nontemporal_load(bt + l * kp);
+      const double2 a = fc[l * stride];
+      const double2 c = wrow[-(int)(l - la)];
+      accum_term(a, c, b, qr, qi);
+    }
+    if (lr < lr_end) {  // the next row's window into the other buffer
+      window_store(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
   }
   if (kc < NQ) part[(long)slab * NQ + kc] = make_double2(qr, qi);
 }
@@ -289,28 +393,78 @@ int launch_build_dup(int N, const double* d_fc, int fc_stride, double* d_dup, hi
   return kOk;
 }
 
-int launch_calq_plain64(int N, const double* d_B, const double* d_dup, const double* d_fc,
-                        int fc_stride, double* d_part, int nslab, hipStream_t s) {
+template <typename T>
+int launch_plain_stream(const char* what, int N, const T* d_B, const double* d_dup, const double* d_fc,
+                        int fc_stride, double* d_part, int nslab, int kernel, hipStream_t s) {
   Geometry g(N);
-  EDB_REQUIRE(nslab >= 1 && nslab <= 4096 && nslab <= g.NQ, "calq_plain64: bad slab count");
-  int ntile = (int)((g.NQ + 63) / 64);
-  hipLaunchKernelGGL((calq_plain64_kernel<8, double>), dim3(ntile, nslab), dim3(64), 0, s, g.M, g.NQ, d_B,
-                     (const double2*)d_dup, (const double2*)d_fc, fc_stride, (double2*)d_part, nslab);
+  EDB_REQUIRE(nslab >= 1 && nslab <= 4096 && nslab <= g.NQ, std::string(what) + ": bad slab count");
+  EDB_REQUIRE(kernel >= 0 && kernel < kPlainKernels, std::string(what) + ": unknown kernel variant");
+  const int ntile = (int)((g.NQ + 63) / 64);
+  const dim3 grid(ntile, nslab);
+  const size_t lds = kernel == 0 ? 0 : sizeof(double2) * 2 * win_rows(g.M) * (2 * g.M - 1);
+  EDB_REQUIRE(win_rows(g.M) * (2 * g.M - 1) <= kWinMaxLoads * 64, std::string(what) + ": window too large");
+  auto args = [&](auto fn) {
+    hipLaunchKernelGGL(fn, grid, dim3(64), lds, s, g.M, g.NQ, d_B, (const double2*)d_dup, (const double2*)d_fc,
+                       fc_stride, (double2*)d_part, nslab);
+  };
+  const int nst = win_loads(g.M);
+  auto lds_kernel = [&](auto u) {
+    constexpr int U = decltype(u)::value;
+    if (nst == 3) args(calq_plain64_lds_kernel<U, 3, T>);
+    else if (nst == 4) args(calq_plain64_lds_kernel<U, 4, T>);
+    else args(calq_plain64_lds_kernel<U, kWinMaxLoads, T>);
+  };
+  switch (kernel) {
+    case 0: args(calq_plain64_kernel<8, T>); break;
+    case 1: lds_kernel(std::integral_constant<int, 4>()); break;
+    case 2: lds_kernel(std::integral_constant<int, 2>()); break;
+    default: lds_kernel(std::integral_constant<int, 8>()); break;
+  }
   EDB_HIP_TRY(hipGetLastError());
   return kOk;
+}
+
+// Resident workgroups per CU of the stream kernel `kernel` (one wave each):
+// slab counts up to this many per CU run in a single round.
+template <typename T>
+int plain_occupancy(int N, int kernel) {
+  Geometry g(N);
+  const size_t lds = kernel == 0 ? 0 : sizeof(double2) * 2 * win_rows(g.M) * (2 * g.M - 1);
+  const int nst = win_loads(g.M);
+  int n = 0;
+  auto occ = [&](auto fn) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64, lds) != hipSuccess) n = 0;
+  };
+  auto lds_kernel = [&](auto u) {
+    constexpr int U = decltype(u)::value;
+    if (nst == 3) occ(calq_plain64_lds_kernel<U, 3, T>);
+    else if (nst == 4) occ(calq_plain64_lds_kernel<U, 4, T>);
+    else occ(calq_plain64_lds_kernel<U, kWinMaxLoads, T>);
+  };
+  switch (kernel) {
+    case 0: occ(calq_plain64_kernel<8, T>); break;
+    case 1: lds_kernel(std::integral_constant<int, 4>()); break;
+    case 2: lds_kernel(std::integral_constant<int, 2>()); break;
+    default: lds_kernel(std::integral_constant<int, 8>()); break;
+  }
+  (void)hipGetLastError();
+  return n;
+}
+
+int plain_blocks_per_cu(int N, int kernel, bool fp32) {
+  return fp32 ? plain_occupancy<float>(N, kernel) : plain_occupancy<double>(N, kernel);
+}
+
+int launch_calq_plain64(int N, const double* d_B, const double* d_dup, const double* d_fc,
+                        int fc_stride, double* d_part, int nslab, int kernel, hipStream_t s) {
+  return launch_plain_stream("calq_plain64", N, d_B, d_dup, d_fc, fc_stride, d_part, nslab, kernel, s);
 }
 
 // Same evaluation with the FP32 copy of beta (products and sums stay FP64):
 // half the bytes of the HBM stream.
 int launch_calq_plain32(int N, const float* d_B, const double* d_dup, const double* d_fc, int fc_stride,
-                        double* d_part, int nslab, hipStream_t s) {
-  Geometry g(N);
-  EDB_REQUIRE(nslab >= 1 && nslab <= 4096 && nslab <= g.NQ, "calq_plain32: bad slab count");
-  int ntile = (int)((g.NQ + 63) / 64);
-  hipLaunchKernelGGL((calq_plain64_kernel<8, float>), dim3(ntile, nslab), dim3(64), 0, s, g.M, g.NQ, d_B,
-                     (const double2*)d_dup, (const double2*)d_fc, fc_stride, (double2*)d_part, nslab);
-  EDB_HIP_TRY(hipGetLastError());
-  return kOk;
+                        double* d_part, int nslab, int kernel, hipStream_t s) {
+  return launch_plain_stream("calq_plain32", N, d_B, d_dup, d_fc, fc_stride, d_part, nslab, kernel, s);
 }
 
 int build_orbit_groups(int N, std::vector<int>* out) {

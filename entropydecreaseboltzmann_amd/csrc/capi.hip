@@ -198,12 +198,12 @@ int plain_column(edb_qcalc* q, Workspace* ws, const double* d_fc, int fc_stride,
   EDB_TRY(launch_build_dup(q->N, d_fc, fc_stride, ws->d_dup, s));
   if (ev0) EDB_HIP_TRY(hipEventRecord(ev0, s));
   if (src == kPlainFP32)
-    EDB_TRY(launch_calq_plain32(q->N, q->d_plain32, ws->d_dup, d_fc, fc_stride, ws->d_part, ns, s));
+    EDB_TRY(launch_calq_plain32(q->N, q->d_plain32, ws->d_dup, d_fc, fc_stride, ws->d_part, ns, q->pkern32, s));
   else if (src == kPlainCompressed)
     EDB_TRY(launch_calq_plain_otf(q->N, q->tables.rval, q->d_ftab, q->d_sb, ws->d_dup, d_fc, fc_stride,
                                   q->d_kgrp, q->n_kgrp, ws->d_part, ns, s));
   else
-    EDB_TRY(launch_calq_plain64(q->N, q->d_plain, ws->d_dup, d_fc, fc_stride, ws->d_part, ns, s));
+    EDB_TRY(launch_calq_plain64(q->N, q->d_plain, ws->d_dup, d_fc, fc_stride, ws->d_part, ns, q->pkern, s));
   if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
   EDB_TRY(launch_reduce_plain(q->N, ws->d_part, ns, d_q, q_stride, s));
   return kOk;
@@ -263,8 +263,9 @@ int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out) {
   q->L = L;
   q->g = Geometry(N);
   q->ntile = (int)((q->g.NQ + 63) / 64);
-  // enough l-slabs to give every SIMD several waves of the HBM stream
-  q->nslab = std::max(1, std::min(64, (cu_count() * 12 + q->ntile - 1) / q->ntile));
+  // enough l-slabs to give every SIMD several waves of the HBM stream (~6 per
+  // SIMD: the LDS-window kernel's best at N = 24, 0.18 ms)
+  q->nslab = std::max(1, std::min(64, (cu_count() * 24 + q->ntile - 1) / q->ntile));
   q->nslab = (int)std::min<long>(q->nslab, q->g.NQ);
   // the FP32-beta stream is half as long and wants ~4x the waves in flight (measured at N = 24),
   // capped at 64 slabs like the FP64 stream so small grids do not drown in partial sums
@@ -642,7 +643,18 @@ static int time_plain_src(edb_qcalc* q, int reps, double* ms, int src) {
       st = kHipError;
   }
   if (st == kOk && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = kHipError;
+  // first call (scratch allocation), then ~20 ms of back-to-back calls so that
+  // the timed ones run at the device's sustained clock
   if (st == kOk) st = plain_column(q, &q->ws, fc, 1, dq, 1, q->stream, nullptr, nullptr, src);
+  if (st == kOk && hipEventRecord(e0, q->stream) != hipSuccess) st = kHipError;
+  if (st == kOk) st = plain_column(q, &q->ws, fc, 1, dq, 1, q->stream, nullptr, nullptr, src);
+  if (st == kOk && hipEventRecord(e1, q->stream) != hipSuccess) st = kHipError;
+  if (st == kOk && hipEventSynchronize(e1) != hipSuccess) st = kHipError;
+  float est = 0.f;
+  if (st == kOk && hipEventElapsedTime(&est, e0, e1) != hipSuccess) st = kHipError;
+  const int warm = (int)std::min(400.0, std::max(2.0, std::ceil(20.0 / std::max((double)est, 1e-3))));
+  for (int r = 0; st == kOk && r < warm; ++r)
+    st = plain_column(q, &q->ws, fc, 1, dq, 1, q->stream, nullptr, nullptr, src);
   if (st == kOk && hipEventRecord(e0, q->stream) != hipSuccess) st = kHipError;
   for (int r = 0; st == kOk && r < reps; ++r)
     st = plain_column(q, &q->ws, fc, 1, dq, 1, q->stream, nullptr, nullptr, src);
@@ -672,6 +684,25 @@ int edb_qcalc_tune(edb_qcalc* q, int batched_waves, int plain_slabs) {
   std::lock_guard<std::mutex> lk(q->mu);
   q->nwaves = batched_waves;
   if (plain_slabs > 0) q->nslab = (int)std::min<long>(plain_slabs, q->g.NQ);  // d_part grows on use
+  return EDB_OK;
+}
+
+int edb_qcalc_tune_plain(edb_qcalc* q, int fp32, int kernel, int slabs) {
+  EDB_REQUIRE(q, "null handle");
+  EDB_REQUIRE(kernel >= -1 && kernel < kPlainKernels, "tune_plain: unknown kernel variant");
+  EDB_REQUIRE(slabs >= 0 && slabs <= 4096, "tune_plain: slabs out of range");
+  std::lock_guard<std::mutex> lk(q->mu);
+  int& kern = fp32 ? q->pkern32 : q->pkern;
+  int& ns = fp32 ? q->nslab32 : q->nslab;
+  if (kernel >= 0) kern = kernel;
+  if (slabs > 0) ns = (int)std::min<long>(slabs, q->g.NQ);  // the partial sums grow on use
+  return EDB_OK;
+}
+
+int edb_qcalc_plain_schedule(const edb_qcalc* q, int fp32, int* kernel, int* slabs) {
+  EDB_REQUIRE(q, "null handle");
+  if (kernel) *kernel = fp32 ? q->pkern32 : q->pkern;
+  if (slabs) *slabs = fp32 ? (q->nslab32 > 0 ? q->nslab32 : q->nslab) : q->nslab;
   return EDB_OK;
 }
 
@@ -728,34 +759,68 @@ int edb_qcalc_autotune(edb_qcalc* q, int batch, int* batched_waves, int* plain_s
     if (bsched == EDB_SCHED_QUADS) q->nblocks_quad = bcount;
     else q->nwaves = bcount;
   }
-  if (q->d_plain) {
+  // l-slab candidates of a stream variant: waves per CU from 8 to 64, and the
+  // counts whose workgroups all fit on the device at once (one round, no
+  // tail of late workgroups) at full, 3/4 and 1/2 of its occupancy
+  auto slab_candidates = [&](int kern, bool fp32, std::vector<int>* out) {
+    out->clear();
+    for (int waves_per_cu : {8, 12, 16, 20, 24, 28, 32, 40, 48, 64})
+      out->push_back((cu * waves_per_cu + q->ntile - 1) / q->ntile);
+    const int occ = plain_blocks_per_cu(q->N, kern, fp32);
+    for (int f4 : {4, 3, 2})
+      if (occ > 0) out->push_back(cu * occ * f4 / 4 / q->ntile);
+    for (int& v : *out) v = (int)std::min<long>(q->g.NQ, std::max(1, std::min(4096, v)));
+    std::sort(out->begin(), out->end());
+    out->erase(std::unique(out->begin(), out->end()), out->end());
+  };
+  if (q->d_plain) {  // stream variant x l-slabs (waves in flight); the best 3 timed again
+    std::vector<std::pair<double, std::pair<int, int>>> cand;
+    std::vector<int> slabs;
+    for (int kern = 0; kern < kPlainKernels; ++kern) {
+      q->pkern = kern;
+      slab_candidates(kern, false, &slabs);
+      for (int ns : slabs) {
+        q->nslab = ns;
+        double ms = 0.;
+        EDB_TRY(edb_qcalc_time_plain(q, 5, &ms));
+        cand.push_back({ms, {kern, q->nslab}});
+      }
+    }
+    std::sort(cand.begin(), cand.end());
     double best = 1e30;
-    int bs = q->nslab;
-    for (int waves_per_cu : {8, 16, 24, 32, 48, 64}) {
-      int ns = std::max(1, std::min(4096, (cu * waves_per_cu + q->ntile - 1) / q->ntile));
-      EDB_TRY(edb_qcalc_tune(q, q->nwaves, ns));
+    int bk = cand[0].second.first, bs = cand[0].second.second;
+    for (size_t i = 0; i < cand.size() && i < 3; ++i) {
+      q->pkern = cand[i].second.first;
+      q->nslab = cand[i].second.second;
       double ms = 0.;
-      EDB_TRY(edb_qcalc_time_plain(q, 3, &ms));
+      EDB_TRY(edb_qcalc_time_plain(q, 20, &ms));
       if (ms < best) {
         best = ms;
+        bk = q->pkern;
         bs = q->nslab;
       }
     }
-    EDB_TRY(edb_qcalc_tune(q, q->nwaves, bs));
+    q->pkern = bk;
+    q->nslab = bs;
   }
   if (q->d_plain32) {  // the FP32-beta stream is shorter and wants more slabs in flight
     double best = 1e30;
-    int bs = 0;
-    for (int waves_per_cu : {16, 32, 48, 64, 96}) {
-      q->nslab32 = std::max(1, (int)std::min<long>({4096L, q->g.NQ / 16, (cu * waves_per_cu + q->ntile - 1) / q->ntile}));
-      double ms = 0.;
-      EDB_TRY(time_plain_src(q, 3, &ms, kPlainFP32));
-      if (ms < best) {
-        best = ms;
-        bs = q->nslab32;
+    int bs = 0, bk = q->pkern32;
+    for (int kern = 0; kern < kPlainKernels; ++kern) {
+      q->pkern32 = kern;
+      for (int waves_per_cu : {16, 32, 48, 64, 96}) {
+        q->nslab32 = std::max(1, (int)std::min<long>({4096L, q->g.NQ / 16, (cu * waves_per_cu + q->ntile - 1) / q->ntile}));
+        double ms = 0.;
+        EDB_TRY(time_plain_src(q, 3, &ms, kPlainFP32));
+        if (ms < best) {
+          best = ms;
+          bs = q->nslab32;
+          bk = kern;
+        }
       }
     }
     q->nslab32 = bs;
+    q->pkern32 = bk;
   }
   if (batched_waves)
     *batched_waves = q->schedule == EDB_SCHED_QUADS ? q->nblocks_quad * quad_waves_per_block(q->N) : q->nwaves;
@@ -855,16 +920,17 @@ static int efix_run(edb_efix_state* st, int mode, long Nf, double* f_host, bool 
                     double* result) {
   EDB_REQUIRE(st && f_host, "EntropyFix: null argument");
   EDB_REQUIRE(Nf == st->Nt, "EntropyFix: Nf != Nt (the reference asserts Nf == Nt)");
-  double *df = nullptr, *ds = nullptr;
+  double *df = nullptr, *ds = nullptr, *dt = nullptr;
   int s = dmalloc(&df, (size_t)Nf);
   if (s == kOk) s = dmalloc(&ds, 3);
+  if (s == kOk && mode != 0) s = dmalloc(&dt, (size_t)Nf);
   double hst[3] = {st->average, (double)st->average_flag, 0.};
   if (s == kOk && (hipMemcpy(df, f_host, sizeof(double) * Nf, hipMemcpyHostToDevice) != hipSuccess ||
                    hipMemcpy(ds, hst, sizeof(hst), hipMemcpyHostToDevice) != hipSuccess)) {
     set_error("EntropyFix: copy failed");
     s = kHipError;
   }
-  if (s == kOk) s = launch_efix(mode, Nf, df, st->eps, aim, ds, nullptr);
+  if (s == kOk) s = launch_efix(mode, Nf, df, st->eps, aim, ds, mode != 0 ? dt : df, nullptr);
   if (s == kOk && hipMemcpy(hst, ds, sizeof(hst), hipMemcpyDeviceToHost) != hipSuccess) s = kHipError;
   if (s == kOk && write_back &&
       hipMemcpy(f_host, df, sizeof(double) * Nf, hipMemcpyDeviceToHost) != hipSuccess)
@@ -878,6 +944,7 @@ static int efix_run(edb_efix_state* st, int mode, long Nf, double* f_host, bool 
   }
   dfree(&df);
   dfree(&ds);
+  dfree(&dt);
   return s;
 }
 

@@ -80,6 +80,8 @@ struct edb_qcalc {
   int ntile = 0;
   int nslab = 16;    // l-slabs of the FP64 plain kernel
   int nslab32 = 0;   // l-slabs of the FP32-beta plain kernel (0: nslab)
+  int pkern = 1;     // stored-beta stream variant of the FP64 plain kernel (launch_calq_plain64)
+  int pkern32 = 1;   // same for the FP32-beta plain kernel
   // scratch of the handle's own evaluations (calQ*, time_*, autotune)
   edb::Workspace ws;
   // paired FP32

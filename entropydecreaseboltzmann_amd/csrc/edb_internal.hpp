@@ -119,10 +119,18 @@ int launch_calq_plain_otf(int N, double rval, const double* d_T, const double* d
 
 // plain FP64 Q (calq_plain.hip)
 int launch_build_dup(int N, const double* d_fc, int fc_stride, double* d_dup, hipStream_t s);
+// Kernel variants of the stored-beta stream: 0 groups of 8 terms, f_[k-l]
+// gathered from the difference table with one vector load per term; 1, 2, 3
+// f_[k-l] read from a per-l-row window of the table staged in LDS, beta in
+// groups of 4, 2, 8 terms.  All four sum the terms in the same order with the
+// same arithmetic: bitwise equal results.
+constexpr int kPlainKernels = 4;
+// Resident one-wave workgroups per CU of a stream variant (0 if unknown).
+int plain_blocks_per_cu(int N, int kernel, bool fp32);
 int launch_calq_plain32(int N, const float* d_B, const double* d_dup, const double* d_fc, int fc_stride,
-                        double* d_part, int nslab, hipStream_t s);
+                        double* d_part, int nslab, int kernel, hipStream_t s);
 int launch_calq_plain64(int N, const double* d_B, const double* d_dup, const double* d_fc,
-                        int fc_stride, double* d_part, int nslab, hipStream_t s);
+                        int fc_stride, double* d_part, int nslab, int kernel, hipStream_t s);
 int launch_reduce_plain(int N, const double* d_part, int nslab, double* d_q, int q_stride,
                         hipStream_t s);
 

@@ -186,21 +186,59 @@ __device__ inline double flogf_dev(double f) { return fabs(f) < 1.0e-6 ? 0. : f 
 // EntropyFix.h:124-161).  Early exit when Aim-eps < H < Aim, else
 // f <- f_{betaR}.  Every thread sees the same block sums, so the loop is
 // uniform.  f is rewritten in place.
-__device__ void fixsol_block(double* f, long N3, double aim, double eps, double avg, double* sh) {
+//
+// The blend and the final update round each product and sum separately, as
+// the reference's loops do (EntropyFix.h:219-222, 249-252).  With `term`
+// non-null every entropy sum runs in the reference's index order (the terms
+// are formed in parallel into term[0..N3), then one thread adds them up), so
+// the bisection takes the reference's decisions wherever its margin exceeds
+// the difference of the device's log from the host's; with `term` null the
+// sums are parallel block reductions.
+__device__ double seq_sum(const double* term, long n, double* sh) {
+  __syncthreads();  // every thread's terms written
+  if (threadIdx.x == 0) {
+    double s = 0.;
+    for (long i = 0; i < n; ++i) s += term[i];
+    sh[0] = s;
+  }
+  __syncthreads();
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ inline double blend(double betaFbar, double betaMid, double f) {
+#pragma clang fp contract(off)
+  return betaFbar + (1. - betaMid) * f;
+}
+
+__device__ inline double blend_final(double betaR, double betaFbar, double f) {
+#pragma clang fp contract(off)
+  return f * (1.0 - betaR) + betaFbar;
+}
+
+__device__ inline double xlogx_above(double f, double eps) { return f > eps ? f * log(f) : 0.; }
+
+__device__ void fixsol_block(double* f, long N3, double aim, double eps, double avg, double* sh,
+                             double* term = nullptr) {
   const int t = threadIdx.x;
   double betaL = 0.0, betaR = 1.0;
   while ((betaR - betaL) > eps) {
     const double betaMid = 0.5 * (betaL + betaR);
     const double betaFbar = betaMid * avg;
-    double s = 0.;
+    double sum;
+    if (term) {
+      for (long g = t; g < N3; g += kRedThreads) term[g] = xlogx_above(blend(betaFbar, betaMid, f[g]), eps);
+      sum = seq_sum(term, N3, sh);
+    } else {
+      double s = 0.;
 #pragma unroll 4
-    for (long g = t; g < N3; g += kRedThreads) {
-      double fm = betaFbar + (1. - betaMid) * f[g];
-      if (fm > eps) s += fm * log(fm);
+      for (long g = t; g < N3; g += kRedThreads) s += xlogx_above(blend(betaFbar, betaMid, f[g]), eps);
+      sum = block_sum(s, sh);
     }
-    const double entMid = block_sum(s, sh) / (double)N3 - avg;
+    const double entMid = sum / (double)N3 - avg;
     if ((entMid < aim) && ((aim - entMid) < eps)) {
-      for (long g = t; g < N3; g += kRedThreads) f[g] = betaFbar + (1. - betaMid) * f[g];
+      for (long g = t; g < N3; g += kRedThreads) f[g] = blend(betaFbar, betaMid, f[g]);
       __syncthreads();
       return;
     }
@@ -208,7 +246,7 @@ __device__ void fixsol_block(double* f, long N3, double aim, double eps, double 
     else betaL = betaMid;
   }
   const double betaFbar = betaR * avg;
-  for (long g = t; g < N3; g += kRedThreads) f[g] = f[g] * (1.0 - betaR) + betaFbar;
+  for (long g = t; g < N3; g += kRedThreads) f[g] = blend_final(betaR, betaFbar, f[g]);
   __syncthreads();
 }
 
@@ -344,28 +382,28 @@ __global__ void bump_kernel(int* p) {
 // The EntropyFix members on one contiguous vector (one block):
 // mode 0 setAverage, 1 EntropyCal, 2 FixSol (EntropyFix.h:65-253).
 // st[0] = average, st[1] = flag, st[2] = result entropy.
+// The reference's EntropyFix on one vector (EntropyFix.h:51-253), every sum in
+// the reference's index order: mode 0 setAverage, 1 EntropyCal (setting the
+// average on first use), 2 FixSol.  st = {average, average set, result};
+// term: N3 doubles of scratch.
 __global__ __launch_bounds__(512) void efix_kernel(int mode, long N3, double* f, double eps, double aim,
-                                                   double* st) {
+                                                   double* st, double* term) {
   __shared__ double sh[kRedThreads];
   const int t = threadIdx.x;
   double avg = st[0];
   if (mode == 0 || st[1] == 0.) {
-    double s = 0.;
-    for (long g = t; g < N3; g += kRedThreads) s += f[g];
-    avg = block_sum(s, sh) / (double)N3;
+    avg = seq_sum(f, N3, sh) / (double)N3;
     if (t == 0) {
       st[0] = avg;
       st[1] = 1.;
     }
   }
   if (mode == 1) {
-    double s = 0.;
-    for (long g = t; g < N3; g += kRedThreads)
-      if (f[g] > eps) s += f[g] * log(f[g]);
-    double r = block_sum(s, sh) / (double)N3 - avg;
+    for (long g = t; g < N3; g += kRedThreads) term[g] = xlogx_above(f[g], eps);
+    const double r = seq_sum(term, N3, sh) / (double)N3 - avg;
     if (t == 0) st[2] = r;
   } else if (mode == 2) {
-    fixsol_block(f, N3, aim, eps, avg, sh);
+    fixsol_block(f, N3, aim, eps, avg, sh, term);
   }
 }
 
@@ -580,9 +618,11 @@ int launch_bump(int* p, hipStream_t s) {
   return kOk;
 }
 
-int launch_efix(int mode, long N3, double* d_f, double eps, double aim, double* d_st, hipStream_t s) {
+int launch_efix(int mode, long N3, double* d_f, double eps, double aim, double* d_st, double* d_term,
+                hipStream_t s) {
   EDB_REQUIRE(N3 >= 1, "EntropyFix: Nf must be positive");
-  hipLaunchKernelGGL(efix_kernel, dim3(1), dim3(kRedThreads), 0, s, mode, N3, d_f, eps, aim, d_st);
+  EDB_REQUIRE(d_term, "EntropyFix: no term scratch");
+  hipLaunchKernelGGL(efix_kernel, dim3(1), dim3(kRedThreads), 0, s, mode, N3, d_f, eps, aim, d_st, d_term);
   EDB_HIP_TRY(hipGetLastError());
   return kOk;
 }

@@ -79,7 +79,8 @@ int launch_stage_real(long n, int batch, int ld, const int* flag, const double* 
                       hipStream_t s);
 // Real part of column col of a [n][ld] complex array into out[n].
 int launch_extract_col(long n, int ld, int col, const double* a, double* out, hipStream_t s);
-int launch_efix(int mode, long N3, double* d_f, double eps, double aim, double* d_st, hipStream_t s);
+int launch_efix(int mode, long N3, double* d_f, double eps, double aim, double* d_st, double* d_term,
+                hipStream_t s);
 int launch_moments(int N, int batch, int ld, double L, const double* F, double* out, hipStream_t s);
 int launch_bkw_error(int N, int batch, int ld, double L, double K, const double* F, double* out,
                      hipStream_t s);

@@ -179,6 +179,17 @@ int edb_qcalc_tune(edb_qcalc* q, int batched_waves, int plain_slabs);
  * the fastest; returns them.  No reference counterpart. */
 int edb_qcalc_autotune(edb_qcalc* q, int batch, int* batched_waves, int* plain_slabs);
 
+/* Plain-kernel knobs of the stored-beta stream, FP64 beta (fp32 == 0) or its
+ * FP32 copy (fp32 != 0): the kernel variant (0: f_[k-l] gathered from the
+ * difference table by one vector load per term, beta in groups of 8; 1, 2,
+ * 3: f_[k-l] from a per-l-row window of the table staged in LDS, beta in
+ * groups of 4, 2, 8; -1 keeps it) and
+ * the l-slab count (0 keeps it).  Every variant sums in the same order with
+ * the same arithmetic, so results are bitwise equal.  The autotuner times
+ * them.  No reference counterpart. */
+int edb_qcalc_tune_plain(edb_qcalc* q, int fp32, int kernel, int slabs);
+int edb_qcalc_plain_schedule(const edb_qcalc* q, int fp32, int* kernel, int* slabs);
+
 /* Work schedule of the batched kernel.  EDB_SCHED_ROWS: stream-K over
  * (k-row, instance group, l-row) steps with `count` persistent waves.
  * EDB_SCHED_QUADS: workgroups of 4 neighbouring k-rows walking the union of
