@@ -25,7 +25,7 @@
 // accumulators (blocked compensation).
 //
 // Load balance.  The kernel is persistent: nwaves waves split the flattened
-// sequence of (k-row, k3-half, instance group, l-row) steps into equal
+// sequence of (instance group, k-row, k3-half, l-row) steps into equal
 // contiguous ranges (stream-K).  Items a wave covers completely are written
 // directly; the partial items at each range end go to two per-wave slots,
 // which a fixup kernel sums in wave order.  Results are deterministic.
@@ -177,7 +177,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(wpe_of(N), 
     BatchedArgs A) {
   constexpr int n = (N - 1) / 2, M = 2 * n + 1, KPT = (M + KS - 1) / KS;
   const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  // Workgroups are dealt to the 8 XCDs in turn; logical block (b % 8) *
+  // per_xcd + b / 8 (the grid is a multiple of 8) gives each XCD a
+  // contiguous run of the ranges, and with the items in instance-group-major
+  // order that run stays within one or two groups' f (6 MB each at N = 24)
+  // instead of all of them, so more of its row loads hit the XCD's L2.
+  const int per_xcd = gridDim.x / 8;
+  const int lb = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  const int w = __builtin_amdgcn_readfirstlane(lb * (blockDim.x >> 6) + (threadIdx.x >> 6));
   if (w >= A.nwaves) return;
   const long a0 = A.total * w / A.nwaves, b0 = A.total * (w + 1) / A.nwaves;
   const clong_p prefix = (clong_p)A.prefix;
@@ -198,7 +205,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(wpe_of(N), 
   while (pos < b0) {
     const long p0 = prefix[it], p1 = prefix[it + 1];
     const long s0 = pos - p0, s1 = (b0 < p1 ? b0 : p1) - p0;
-    const int r = it / (KS * A.groups), h = (it / A.groups) % KS, g = it % A.groups;
+    const int per_g = A.nitems / A.groups;  // items of one instance group: (row, k3 part)
+    const int g = it / per_g, r = (it % per_g) / KS, h = it % KS;
     const int rk1 = rowinfo[4 * r], rk2 = rowinfo[4 * r + 1], rblk0 = rowinfo[4 * r + 2];
     double ar[KPT], ai[KPT];
 #pragma unroll
@@ -274,7 +282,8 @@ __global__ __launch_bounds__(256) void calq_fixup_kernel(BatchedArgs A, int N, i
     soff[i] = (long)(2 * w + slot) * KPT * 64;
   }
   __syncthreads();
-  const int r = it / (KS * A.groups), h = (it / A.groups) % KS, g = it % A.groups;
+  const int per_g = A.nitems / A.groups;
+  const int g = it / per_g, r = (it % per_g) / KS, h = it % KS;
   const int4 ri = A.rowinfo[r];
   const bool mirror = A.hermitian && !(ri.x == n && ri.y == n);
   // one thread per (k3 entry, lane); the covering waves' loads are issued together
@@ -360,7 +369,9 @@ __global__ __launch_bounds__(64 * kQuad * KS) __attribute__((amdgpu_waves_per_eu
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int j = wave % kQuad, h = wave / kQuad;
-  const int blk = blockIdx.x;
+  // XCD-contiguous logical block (the grid is a multiple of 8), as the rows kernel
+  const int per_xcd = gridDim.x / 8;
+  const int blk = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (blk >= A.nblocks) return;
   const long a0 = A.total * blk / A.nblocks, b0 = A.total * (blk + 1) / A.nblocks;
   const clong_p prefix = (clong_p)A.prefix;
@@ -384,7 +395,8 @@ __global__ __launch_bounds__(64 * kQuad * KS) __attribute__((amdgpu_waves_per_eu
   while (pos < b0) {
     const long p0 = prefix[it], p1 = prefix[it + 1];
     const long s0 = pos - p0, s1 = (b0 < p1 ? b0 : p1) - p0;
-    const int qd = it / A.groups, g = it % A.groups;
+    const int per_g = A.nitems / A.groups;  // quads of one instance group
+    const int g = it / per_g, qd = it % per_g;
     const int nrows = quadinfo[4 * qd + 2], uoff = quadinfo[4 * qd + 3];
     double ar[KPT], ai[KPT];
 #pragma unroll
@@ -458,7 +470,8 @@ __global__ __launch_bounds__(256 * 2) void calq_quad_fixup_kernel(QuadArgs A, in
     soff[i] = (long)(2 * b + slot) * WPB * KPT * 64;
   }
   __syncthreads();
-  const int qd = it / A.groups, g = it % A.groups;
+  const int per_g = A.nitems / A.groups;
+  const int g = it / per_g, qd = it % per_g;
   const int nrows = A.quadinfo[qd].z;
   // one thread per (wave of the block, k3 entry, lane)
   for (int e = threadIdx.x; e < WPB * KPT * 64; e += blockDim.x) {
@@ -602,10 +615,10 @@ int build_pair_plan(int N, std::vector<int4>* rowinfo, std::vector<uint32_t>* sc
 
 int batched_prefix(const std::vector<int4>& rowinfo, int nrows, int groups, int ks,
                    std::vector<long>* prefix) {
-  prefix->assign(1, 0);
-  for (int r = 0; r < nrows; ++r)
-    for (int h = 0; h < ks; ++h)
-      for (int g = 0; g < groups; ++g) prefix->push_back(prefix->back() + rowinfo[r].w);
+  prefix->assign(1, 0);  // items in (instance group, row, k3 part) order
+  for (int g = 0; g < groups; ++g)
+    for (int r = 0; r < nrows; ++r)
+      for (int h = 0; h < ks; ++h) prefix->push_back(prefix->back() + rowinfo[r].w);
   return kOk;
 }
 
@@ -632,7 +645,8 @@ int launch_calq_batched_impl(int N, const float* d_B, long nbeta, const uint32_t
   A.hermitian = hermitian;
   const int wpb = 4;
   if (ev0) EDB_HIP_TRY(hipEventRecord(ev0, s));
-  hipLaunchKernelGGL(fn, dim3((nwaves + wpb - 1) / wpb), dim3(64 * wpb), 0, s, A);
+  const int nblk = ((nwaves + wpb - 1) / wpb + 7) / 8 * 8;  // a multiple of the 8 XCDs
+  hipLaunchKernelGGL(fn, dim3(nblk), dim3(64 * wpb), 0, s, A);
   EDB_HIP_TRY(hipGetLastError());
   if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
   if (nwaves > 1) {
@@ -694,9 +708,9 @@ int build_quad_plan(int N, const std::vector<int4>& rowinfo, const std::vector<u
 }
 
 int quad_prefix(const std::vector<int4>& quadinfo, int nquads, int groups, std::vector<long>* prefix) {
-  prefix->assign(1, 0);
-  for (int qd = 0; qd < nquads; ++qd)
-    for (int g = 0; g < groups; ++g)
+  prefix->assign(1, 0);  // items in (instance group, quad) order
+  for (int g = 0; g < groups; ++g)
+    for (int qd = 0; qd < nquads; ++qd)
       prefix->push_back(prefix->back() + (quadinfo[qd + 1].w - quadinfo[qd].w));
   return kOk;
 }
@@ -725,7 +739,7 @@ int launch_calq_quad_impl(int N, const float* d_B, long nbeta, const uint32_t* d
   A.sink = nullptr;
   A.nbeta = nbeta;
   if (ev0) EDB_HIP_TRY(hipEventRecord(ev0, s));
-  hipLaunchKernelGGL(fn, dim3(nblocks), dim3(64 * wpb), 0, s, A);
+  hipLaunchKernelGGL(fn, dim3((nblocks + 7) / 8 * 8), dim3(64 * wpb), 0, s, A);  // a multiple of the 8 XCDs
   EDB_HIP_TRY(hipGetLastError());
   if (ev1) EDB_HIP_TRY(hipEventRecord(ev1, s));
   if (nblocks > 1) {

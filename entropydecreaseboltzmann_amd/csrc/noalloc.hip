@@ -24,33 +24,40 @@ using namespace edb;
 namespace {
 
 // The schedules' prefix tables in closed form (the same numbers batched_prefix
-// and quad_prefix build on the host): rows r carry w = rowinfo[r].w steps per
-// item and rowinfo[r].z = the steps of all rows before r, so the item
-// it = (r*ks + h)*groups + g starts at z*ks*groups + (it - r*ks*groups)*w;
-// quads carry uoff = quadinfo[qd].w union steps before quad qd, so item
-// qd*groups + g starts at uoff*groups + g*(uoff' - uoff).  One thread per entry.
-__global__ void prefix_fill_kernel(const int4* __restrict__ info, int quads, int nunits, int per_unit,
+// and quad_prefix build on the host).  Rows: row r carries w = rowinfo[r].w
+// steps per item and rowinfo[r].z = the steps of all rows before it; items
+// run (group g, row r, k3 part h), it = g*nunits*ks + r*ks + h, so item it
+// starts at g*T + z*ks + h*w with T = (z + w of the last row) * ks.  Quads:
+// uoff = quadinfo[qd].w union steps before quad qd, items (group g, quad qd),
+// item g*nunits + qd starts at g*U + uoff with U = the union steps of all
+// nunits quads.  One thread per entry.
+__global__ void prefix_fill_kernel(const int4* __restrict__ info, int quads, int nunits, int per_unit, int ks,
                                    long* __restrict__ prefix) {
   const long nitems = (long)nunits * per_unit;
+  const long T = quads ? 0 : ((long)info[nunits - 1].z + info[nunits - 1].w) * ks;
   for (long it = blockIdx.x * (long)blockDim.x + threadIdx.x; it <= nitems; it += (long)gridDim.x * blockDim.x) {
-    const int u = it < nitems ? (int)(it / per_unit) : nunits - 1;
-    const long j = it - (long)u * per_unit;  // per_unit for the end entry
     long v;
     if (quads) {
-      const long u0 = info[u].w, u1 = info[u + 1].w;
-      v = u0 * per_unit + j * (u1 - u0);
+      const long g = it / nunits;
+      const int qd = (int)(it - g * nunits);  // the end entry: g = groups, qd = 0
+      v = g * (long)info[nunits].w + info[qd].w;
     } else {
-      v = (long)info[u].z * per_unit + j * info[u].w;
+      const long per_g = (long)nunits * ks;  // items of one instance group
+      const long g = it / per_g, rem = it - g * per_g;  // the end entry: g = groups, rem = 0
+      const int r = (int)(rem / ks), h = (int)(rem - (long)r * ks);
+      v = g * T + (long)info[r < nunits ? r : nunits - 1].z * ks + h * info[r < nunits ? r : nunits - 1].w;
     }
     prefix[it] = v;
   }
 }
 
-int launch_prefix_fill(const int4* d_info, int quads, int nunits, int per_unit, long* d_prefix, hipStream_t s) {
-  EDB_REQUIRE(nunits >= 1 && per_unit >= 1, "prefix_fill: empty schedule");
+int launch_prefix_fill(const int4* d_info, int quads, int nunits, int per_unit, int ks, long* d_prefix,
+                       hipStream_t s) {
+  EDB_REQUIRE(nunits >= 1 && per_unit >= 1 && ks >= 1, "prefix_fill: empty schedule");
   const long n = (long)nunits * per_unit + 1;
   const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(prefix_fill_kernel, dim3(blocks), dim3(256), 0, s, d_info, quads, nunits, per_unit, d_prefix);
+  hipLaunchKernelGGL(prefix_fill_kernel, dim3(blocks), dim3(256), 0, s, d_info, quads, nunits, per_unit, ks,
+                     d_prefix);
   EDB_HIP_TRY(hipGetLastError());
   return kOk;
 }
@@ -146,7 +153,8 @@ int edb_qcalc_calQ_batched_device_noalloc(edb_qcalc* q, int batch, int ld, int h
   EDB_TRY(noalloc_scratch(&base));
   long* d_prefix = (long*)base;
   double* d_slots = (double*)((char*)base + prefix_bytes);
-  EDB_TRY(launch_prefix_fill(quads ? q->d_quadinfo : q->d_rowinfo, quads, nunits, per_unit, d_prefix, s));
+  EDB_TRY(launch_prefix_fill(quads ? q->d_quadinfo : q->d_rowinfo, quads, nunits, per_unit, batched_ksplit(N),
+                             d_prefix, s));
   if (quads)
     EDB_TRY(launch_calq_quad_impl(N, q->d_pair, q->nblocks, q->d_sched, q->d_rowinfo, q->d_quadinfo, q->d_rowidx,
                                   q->d_ublk, d_prefix, (int)nitems, total, d_fin, ld, groups, hermitian ? 1 : 0,
