@@ -136,7 +136,8 @@ def profile_kernels(edb, N, batch, schedule=None, count=0):
     tb = q.time_batched(batch, True, reps=2)
     tp = q.time_plain(reps=2)
     print(json.dumps({"profile_kernels": True, "N": N, "batch": batch, "batched_ms": tb,
-                      "plain_ms": tp}), flush=True)
+                      "batched_schedule": q.batched_schedule(), "plain_ms": tp,
+                      "plain_schedule": q.plain_schedule()}), flush=True)
     hip.hipFree(src)
     hip.hipFree(dst)
 
@@ -703,7 +704,9 @@ def main():
         ptuned = qp.autotune(0)  # l-slab count of the plain kernel
         pms = qp.time_plain(reps=10)
         out["plain_fp64"] = {"qevals_per_s": round(1e3 / pms, 1), "ms_per_qeval": round(pms, 4),
-                             "plain_slabs": ptuned["plain_slabs"],
+                             "plain_slabs": ptuned["plain_slabs"], "plain_kernel": qp.plain_schedule()[0],
+                             "note": "one Q-evaluation: difference table, the FP64 beta stream "
+                                     "(calq_plain64_lds_kernel when plain_kernel > 0), slab reduction",
                              "beta_gbs": round(M ** 6 * 8 / (pms * 1e-3) / 1e9, 1),
                              "hbm_frac": round(M ** 6 * 8 / (pms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
         out["speedup_vs_plain_fp64"] = round(value / out["plain_fp64"]["qevals_per_s"], 2)

@@ -263,10 +263,11 @@ int edb_qcalc_create(int N, int alpha, double L, edb_qcalc** out) {
   q->L = L;
   q->g = Geometry(N);
   q->ntile = (int)((q->g.NQ + 63) / 64);
-  // enough l-slabs to give every SIMD several waves of the HBM stream (~6 per
-  // SIMD: the LDS-window kernel's best at N = 24, 0.18 ms)
-  q->nslab = std::max(1, std::min(64, (cu_count() * 24 + q->ntile - 1) / q->ntile));
-  q->nslab = (int)std::min<long>(q->nslab, q->g.NQ);
+  // as many l-slabs as the default stream kernel keeps resident at once (one
+  // round of workgroups, no tail): 32 at N = 24, 0.19 ms per Q-evaluation
+  const int occ = plain_blocks_per_cu(N, q->pkern, false);
+  q->nslab = occ > 0 ? cu_count() * occ / q->ntile : (cu_count() * 24 + q->ntile - 1) / q->ntile;
+  q->nslab = (int)std::max(1L, std::min<long>({(long)q->nslab, 4096L, q->g.NQ}));
   // the FP32-beta stream is half as long and wants ~4x the waves in flight (measured at N = 24),
   // capped at 64 slabs like the FP64 stream so small grids do not drown in partial sums
   q->nslab32 = (int)std::max(1L, std::min<long>({64L, q->g.NQ, (cu_count() * 48L + q->ntile - 1) / q->ntile}));
