@@ -125,7 +125,7 @@ class EntropyFix {
 };
 
 // BKW3D drivers on one instance.  The collision operator is the drivers'
-// QCalculator(N0, 0) (BKW.h:204, 467): Jackson filter on (IS_FILTER = 1,
+// QCalculator(N0, 0) (BKW.h:201, 462): Jackson filter on (IS_FILTER = 1,
 // QCalculator.h:34) unless the constructor's `filter` is false.  Initial
 // data: a user function of (t, x, y, z) on x_i = -R + i*2R/N
 // (BKW.h:121-138); by default the BKW
