@@ -22,7 +22,8 @@
 // compile time.  beta rows are prefetched one row ahead into SGPRs behind an
 // explicit wait (scalar loads return out of order, so a later wait would also
 // cover the prefetch).  Each l-row's FP32 partial sums are folded into FP64
-// accumulators (blocked compensation).
+// accumulators (blocked compensation), so every work split gives the same
+// FP32 partials and differs only in the FP64 merge order.
 //
 // Load balance.  The kernel is persistent: nwaves waves split the flattened
 // sequence of (instance group, k-row, k3-half, l-row) steps into equal
@@ -99,7 +100,24 @@ __device__ __forceinline__ float touch_block(const float* __restrict__ blk, int 
   return v;
 }
 
-template <int N, int KPT, int H>
+// f rows of the rows kernel are read with buffer loads: the row's base is a
+// wave-uniform buffer resource, the lane's column a fixed VGPR offset and the
+// mode's offset an SGPR, so no vector instruction computes an address (46
+// fewer VALU per step at N = 24, 0.7% of the launch).  Offsets are 32-bit:
+// the host requires M * Bp * 8 < 2^31 bytes per row.
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const float2* row) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)row, (short)0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ float2 row_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const v2f_t v = __builtin_bit_cast(v2f_t, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  return make_float2(v.x, v.y);
+}
+
+// fa and fcr point at the rows' first entries of this wave's instance group;
+// with BUF they are wave-uniform (the lane is added here), without it they
+// already include the lane.
+template <int N, int KPT, int H, bool BUF>
 __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__ fa,
                                          const float2* __restrict__ fcr, int Bp, float (&pr)[KPT],
                                          float (&pi)[KPT], const float* __restrict__ bnext_blk, int lane,
@@ -108,11 +126,14 @@ __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__
   constexpr int K0 = H * KPT, K1 = (H + 1) * KPT < M ? (H + 1) * KPT : M;
   constexpr int NK = K1 - K0;
   float2 c[M], a[M];
+  const int voff = lane * 8, sstr = Bp * 8;
+  auto ldc = [&](int m) { return BUF ? row_load(row_rsrc(fcr), voff, m * sstr) : fcr[(long)m * Bp]; };
+  auto lda = [&](int m) { return BUF ? row_load(row_rsrc(fa), voff, m * sstr) : fa[(long)m * Bp]; };
 #pragma unroll
-  for (int m = 0; m < M; ++m) c[m] = fcr[(long)m * Bp];
+  for (int m = 0; m < M; ++m) c[m] = ldc(m);
   constexpr int AW = awin_of(N);
 #pragma unroll
-  for (int m = 0; m < (AW > 0 && AW < M ? AW : M); ++m) a[m] = fa[(long)m * Bp];
+  for (int m = 0; m < (AW > 0 && AW < M ? AW : M); ++m) a[m] = lda(m);
   // issued after the f rows, so their waits do not cover it; consumed at the end
   const float tv = bnext_blk ? touch_block<N>(bnext_blk, lane) : 0.f;
 #pragma unroll
@@ -134,18 +155,21 @@ __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__
 #pragma unroll
       for (int j = 0; j < NK; ++j) bnext[j] = bb[(l3 + 1) * M + K0 + j];
     }
-    if (AW > 0 && l3 + AW < M) a[l3 + AW] = fa[(long)(l3 + AW) * Bp];
+    if (AW > 0 && l3 + AW < M) a[l3 + AW] = lda(l3 + AW);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
       const int k3 = K0 + j;
       const int m = cwrapc(N, k3 - l3);
       if (m >= 0) {
+        // the roundings are written out (two products, two FMAs into the
+        // product, two into the sums) so that every kernel built on this step
+        // computes the same FP32 partials whatever code surrounds it
         const float b = bcur[j];
-        const float xr = a[l3].x * c[m].x - a[l3].y * c[m].y;
-        const float xi = a[l3].x * c[m].y + a[l3].y * c[m].x;
-        pr[j] += b * xr;
-        pi[j] += b * xi;
+        const float xr = __builtin_fmaf(a[l3].x, c[m].x, -(a[l3].y * c[m].y));
+        const float xi = __builtin_fmaf(a[l3].x, c[m].y, a[l3].y * c[m].x);
+        pr[j] = __builtin_fmaf(b, xr, pr[j]);
+        pi[j] = __builtin_fmaf(b, xi, pi[j]);
       }
     }
 #pragma unroll
@@ -214,7 +238,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(wpe_of(N), 
       ar[j] = 0.;
       ai[j] = 0.;
     }
-    const float2* fbase = A.fin + (long)g * 64 + lane;
+    const float2* fbase = A.fin + (long)g * 64;  // wave-uniform: row_step adds the lane
     for (long s = s0; s < s1; ++s) {
       const long blk = rblk0 + s;
       const uint32_t e = sched[blk];
@@ -225,8 +249,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(wpe_of(N), 
       float pr[KPT], pi[KPT];
       float tv = 0.f;
       const float* nb = blk + 1 < A.nbeta ? A.B + (blk + 1) * (long)(M * M) : nullptr;
-      if (KS == 1 || h == 0) row_step<N, KPT, 0>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
-      else row_step<N, KPT, (KS > 1 ? 1 : 0)>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
+      if (KS == 1 || h == 0) row_step<N, KPT, 0, true>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
+      else row_step<N, KPT, (KS > 1 ? 1 : 0), true>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
       touched += tv;
 #pragma unroll
       for (int j = 0; j < KPT; ++j) {
@@ -418,8 +442,8 @@ __global__ __launch_bounds__(64 * kQuad * KS) __attribute__((amdgpu_waves_per_eu
         float tv = 0.f;
         // this row's blocks are consecutive in its schedule: b + 1 is its next one
         const float* nb = b + 1 < A.nbeta ? A.B + (b + 1) * (long)(M * M) : nullptr;
-        if (KS == 1 || h == 0) row_step<N, KPT, 0>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
-        else row_step<N, KPT, (KS > 1 ? 1 : 0)>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
+        if (KS == 1 || h == 0) row_step<N, KPT, 0, false>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
+        else row_step<N, KPT, (KS > 1 ? 1 : 0), false>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
         touched += tv;
 #pragma unroll
         for (int t = 0; t < KPT; ++t) {
@@ -634,6 +658,7 @@ int launch_calq_batched_impl(int N, const float* d_B, long nbeta, const uint32_t
   }
   EDB_REQUIRE(Bp % 64 == 0 && groups * 64 <= Bp && groups >= 1, "calq_batched: bad batch padding");
   EDB_REQUIRE(nwaves >= 1 && total >= 1, "calq_batched: empty launch");
+  EDB_REQUIRE((long)(2 * ((N - 1) / 2) + 1) * Bp * 8 < (1L << 31), "calq_batched: batch too large for one launch");
   if (nwaves > total) nwaves = (int)total;
   BatchedArgs A;
   A.B = d_B; A.sched = d_sched; A.rowinfo = d_rowinfo; A.prefix = d_prefix;
