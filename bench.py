@@ -11,7 +11,8 @@ Q-evaluations (4 x 256 Q-evals) + inverse FFT, entropy and fix.
 Prints one JSON line.  value = Q-evals per second over the timed steps
 (inputs resident in HBM).  roofline: the batched Q kernel's executed work
 (10 flops per evaluated pair-term) over its mean launch time, measured with
-HIP events on its own stream during the timed steps, against the FP32 peak;
+the HIP start/stop events of each launch's dispatch on the solver's stream
+during the timed steps, against the FP32 peak;
 the reference's canonical 10 M^6 flops per Q-eval is reported beside it.
 graph_ms_per_step: the same steps replayed from HIP graphs.  plain_fp64: the
 single-instance FP64 path the speedup is quoted against.  p50_s / p99_s /
@@ -679,8 +680,8 @@ def main():
                                    "peak = MI355X's FP32 157.3 TFLOP/s",
                      "flops_note": "executed work: 4 FMA + 2 MUL = 10 flops per evaluated pair-term of "
                                    "the Hermitian-half paired schedule (pair_terms(N) per 64 instances), "
-                                   "over the kernel's mean launch time from HIP events on the solver's "
-                                   "stream in the timed steps; issue_frac = its 6 VALU instructions per "
+                                   "over the kernel's mean launch time from the HIP start/stop events of "
+                                   "each launch's dispatch on the solver's stream in the timed steps; issue_frac = its 6 VALU instructions per "
                                    "pair-term over the dual-issue FP32 VALU rate (1024 SIMDs x 2.4 GHz x "
                                    "2 per 4 cycles)",
                      "traffic_note": "HBM bytes per launch from profiles/calq_batched_n24_b256_traffic.json "
