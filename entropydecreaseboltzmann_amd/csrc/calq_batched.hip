@@ -73,6 +73,7 @@ struct BatchedArgs {
   double2* q;
   double2* slots;
   int* wstart;  // [nwaves] item containing each wave's first step
+  long2* rec;   // [nwaves] first and end step of that item
   float* sink;  // null: keeps the beta prefetch loads alive (never written)
   long nbeta;   // beta blocks allocated (the prefetch never reads past them)
   int hermitian;
@@ -224,7 +225,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(wpe_of(N), 
     else hi = mid;
   }
   int it = lo;
-  if (lane == 0) A.wstart[w] = it;
+  if (lane == 0) {
+    A.wstart[w] = it;
+    A.rec[w] = make_long2(prefix[it], prefix[it + 1]);
+  }
   long pos = a0;
   bool first = true;
   float touched = 0.f;
@@ -296,24 +300,30 @@ __global__ __launch_bounds__(256) void calq_fixup_kernel(BatchedArgs A, int N, i
   const int n = (N - 1) / 2, M = 2 * n + 1, KPT = (M + KS - 1) / KS;
   const int wb = blockIdx.x + 1;
   if (wb >= A.nwaves) return;
-  const long p = range_start(A.total, A.nwaves, wb);
+  // the item containing wave wb's first step p, as the main kernel recorded it
+  const long2 rc = A.rec[wb];
+  const long lo = rc.x, hi = rc.y;
+  const long p = range_start(A.total, A.nwaves, wb), pprev = range_start(A.total, A.nwaves, wb - 1);
+  if (lo == p || pprev > lo) return;
   const int it = A.wstart[wb];
-  const long lo = A.prefix[it], hi = A.prefix[it + 1];
-  if (lo == p || range_start(A.total, A.nwaves, wb - 1) > lo) return;
-  const int wf = wave_of(A.total, A.nwaves, lo), wl = wave_of(A.total, A.nwaves, hi - 1);
+  // wb is the first boundary inside the item, so wave wb - 1 holds its start
+  // (its first slot if it starts there, else its second); the item ends in
+  // the last wave whose range starts before hi
+  const int wf = wb - 1;
+  int wl = wb;
+  while (wl + 1 < A.nwaves && range_start(A.total, A.nwaves, wl + 1) < hi) ++wl;
   const int nw = wl - wf + 1;
-  for (int i = threadIdx.x; i < nw && i < kMaxCover; i += blockDim.x) {
-    const int w = wf + i;
-    const int slot = range_start(A.total, A.nwaves, w) >= lo ? 0 : 1;
-    soff[i] = (long)(2 * w + slot) * KPT * 64;
-  }
+  for (int i = threadIdx.x; i < nw && i < kMaxCover; i += blockDim.x)
+    soff[i] = (long)(2 * (wf + i) + (i == 0 && pprev < lo ? 1 : 0)) * KPT * 64;
   __syncthreads();
   const int per_g = A.nitems / A.groups;
   const int g = it / per_g, r = (it % per_g) / KS, h = it % KS;
   const int4 ri = A.rowinfo[r];
   const bool mirror = A.hermitian && !(ri.x == n && ri.y == n);
-  // one thread per (k3 entry, lane); the covering waves' loads are issued together
-  for (int e = threadIdx.x; e < KPT * 64; e += blockDim.x) {
+  // one thread per (k3 entry, lane), the entries spread over blockIdx.y so
+  // that many blocks' loads are in flight at once; the covering waves' loads
+  // of an entry are issued together
+  for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < KPT * 64; e += blockDim.x * gridDim.y) {
     const int j = e >> 6, lane = e & 63;
     const int k3 = h * KPT + j;
     if (k3 >= M) continue;
@@ -337,8 +347,7 @@ __global__ __launch_bounds__(256) void calq_fixup_kernel(BatchedArgs A, int N, i
       if (i < kMaxCover) {
         base = soff[i];
       } else {
-        const int w = wf + i;
-        base = (long)(2 * w + (range_start(A.total, A.nwaves, w) >= lo ? 0 : 1)) * KPT * 64;
+        base = (long)(2 * (wf + i)) * KPT * 64;  // a wave inside the item: its first slot
       }
       const double2 v = A.slots[base + rel];
       sr += v.x;
@@ -499,8 +508,8 @@ __global__ __launch_bounds__(256 * 2) void calq_quad_fixup_kernel(QuadArgs A, in
   const int per_g = A.nitems / A.groups;
   const int g = it / per_g, qd = it % per_g;
   const int nrows = A.quadinfo[qd].z;
-  // one thread per (wave of the block, k3 entry, lane)
-  for (int e = threadIdx.x; e < WPB * KPT * 64; e += blockDim.x) {
+  // one thread per (wave of the block, k3 entry, lane), spread over blockIdx.y
+  for (int e = blockIdx.y * blockDim.x + threadIdx.x; e < WPB * KPT * 64; e += blockDim.x * gridDim.y) {
     const int wv = e / (KPT * 64), t = (e / 64) % KPT, lane = e & 63;
     const int j = wv % kQuad, h = wv / kQuad;
     const int k3 = h * KPT + t;
@@ -600,6 +609,12 @@ size_t batched_slot_doubles(int N, int nwaves) {
   return (size_t)nwaves * 2 * kpt * 64 * 2;
 }
 
+// All of a launch's scratch: the slots, then per wave the (first, end) steps
+// of its first item and that item's index.
+size_t batched_scratch_doubles(int N, int nwaves) {
+  return batched_slot_doubles(N, nwaves) + 2 * (size_t)nwaves + ((size_t)nwaves + 1) / 2;
+}
+
 // Slots of the quad schedule: 2 per wave of every block (same count as
 // stream-K over nblocks * kQuad * KS waves).
 int quad_waves_per_block(int N) { return kQuad * ksplit_of(N); }
@@ -666,7 +681,8 @@ int launch_calq_batched_impl(int N, const float* d_B, long nbeta, const uint32_t
   A.B = d_B; A.sched = d_sched; A.rowinfo = d_rowinfo; A.prefix = d_prefix;
   A.nitems = nitems; A.groups = groups; A.total = total; A.nwaves = nwaves;
   A.fin = (const float2*)d_fin; A.Bp = Bp; A.q = (double2*)d_q; A.slots = (double2*)d_slots;
-  A.wstart = (int*)(d_slots + batched_slot_doubles(N, nwaves));
+  A.rec = (long2*)(d_slots + batched_slot_doubles(N, nwaves));
+  A.wstart = (int*)(A.rec + nwaves);
   A.sink = nullptr;
   A.nbeta = nbeta;
   A.hermitian = hermitian;
@@ -678,7 +694,8 @@ int launch_calq_batched_impl(int N, const float* d_B, long nbeta, const uint32_t
   else hipLaunchKernelGGL(fn, dim3(nblk), dim3(64 * wpb), 0, s, A);
   EDB_HIP_TRY(hipGetLastError());
   if (nwaves > 1) {
-    hipLaunchKernelGGL(calq_fixup_kernel, dim3(nwaves - 1), dim3(256), 0, s, A, N, ks);
+    const int kpt = (2 * ((N - 1) / 2) + 1 + ks - 1) / ks;
+    hipLaunchKernelGGL(calq_fixup_kernel, dim3(nwaves - 1, (kpt * 64 + 255) / 256), dim3(256), 0, s, A, N, ks);
     EDB_HIP_TRY(hipGetLastError());
   }
   return kOk;
@@ -771,7 +788,9 @@ int launch_calq_quad_impl(int N, const float* d_B, long nbeta, const uint32_t* d
   else hipLaunchKernelGGL(fn, grid, dim3(64 * wpb), 0, s, A);
   EDB_HIP_TRY(hipGetLastError());
   if (nblocks > 1) {
-    hipLaunchKernelGGL(calq_quad_fixup_kernel, dim3(nblocks - 1), dim3(64 * wpb), 0, s, A, N, ks);
+    const int kpt = (2 * ((N - 1) / 2) + 1 + ks - 1) / ks;
+    hipLaunchKernelGGL(calq_quad_fixup_kernel, dim3(nblocks - 1, (wpb * kpt * 64 + 64 * wpb - 1) / (64 * wpb)),
+                       dim3(64 * wpb), 0, s, A, N, ks);
     EDB_HIP_TRY(hipGetLastError());
   }
   return kOk;

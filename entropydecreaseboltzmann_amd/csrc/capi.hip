@@ -75,7 +75,7 @@ int get_prefix(edb_qcalc* q, int groups, int hermitian, PrefixEntry** out) {
 }
 
 int ensure_slots(Workspace* ws, int N, int nwaves) {
-  size_t need = batched_slot_doubles(N, nwaves) + (size_t)(nwaves + 1) / 2;
+  size_t need = batched_scratch_doubles(N, nwaves);
   if (need <= ws->slots_count) return kOk;
   if (ws->d_slots) ws->retired.push_back(ws->d_slots);
   ws->d_slots = nullptr;

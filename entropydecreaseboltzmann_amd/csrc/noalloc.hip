@@ -137,7 +137,7 @@ int edb_qcalc_calQ_batched_device_noalloc(edb_qcalc* q, int batch, int ld, int h
   const size_t prefix_bytes = ((size_t)(nitems + 1) * sizeof(long) + 255) / 256 * 256;
   auto slot_bytes = [&](long count) {
     const size_t w = (size_t)count * wpb;
-    return (batched_slot_doubles(N, (int)w) + (w + 1) / 2) * sizeof(double);
+    return batched_scratch_doubles(N, (int)w) * sizeof(double);
   };
   if (nitems >= (1L << 30) || prefix_bytes + slot_bytes(1) > kNoAllocScratchBytes) {
     set_error("calQ_batched_device_noalloc: batch " + std::to_string(batch) + " at N = " + std::to_string(N) +

@@ -90,6 +90,7 @@ int batched_ksplit(int N);
 // Waves per SIMD the batched kernels are built for at grid size N.
 int batched_waves_per_simd(int N);
 size_t batched_slot_doubles(int N, int nwaves);
+size_t batched_scratch_doubles(int N, int nwaves);
 
 // Conservative moment correction (conserve.hip): table of the 3M-2 axis
 // modes (centred index and the five moment transforms) followed by the 5x5
