@@ -192,6 +192,31 @@ def test_batched_schedules_agree_with_oracle(edb, N, batch, hermitian):
 
 
 @pytest.mark.parametrize("schedule", [0, 1])
+def test_batched_one_step_per_wave(edb, schedule):
+    """Stream-K at its finest split: a persistent count past the number of
+    steps (clamped to one step per wave, or per workgroup for quads), so
+    every item is split and items longer than 64 steps are covered by more
+    than 64 waves (the fixup's per-wave slot table overflows into its
+    fallback); also a single persistent wave (nothing split).  Both agree
+    with the default split to FP64 merge order and with the oracle."""
+    N, L, batch = 16, 8.0, 5
+    fs = hermitize(np.stack([np.fft.fftn(maxwellian_mixture(N, L, 500 + i)) for i in range(batch)]))
+    q = edb.QCalculator(N, 0, L)
+    q.closeFilter()
+    q.genB_batched()
+    q.tune_batched(schedule, 0)
+    base = q.calQ_batched(fs, hermitian=True)
+    for count in (10 ** 6, 1):
+        q.tune_batched(schedule, count)
+        got = q.calQ_batched(fs, hermitian=True)
+        assert np.abs(got - base).max() <= 1e-12 * np.abs(base).max(), count
+    B = oracle.genB(N, 0, L, 0)
+    for i in (0, batch - 1):
+        ref = oracle.calQ(N, B, fs[i])
+        assert np.all(np.abs(got[i] - ref) <= 3e-6 * bound(N, B, fs[i]) + 1e-300), i
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
 def test_largest_grid_batched_matches_plain(edb, schedule):
     """N = 33, the largest compiled batched kernel: FP32 batched (both
     schedules) against the FP64 plain path on the GPU (whose operator the
