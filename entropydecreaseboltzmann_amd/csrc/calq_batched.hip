@@ -15,15 +15,26 @@
 // it is read with scalar loads and used as an SGPR operand -- every beta
 // entry is fetched once per 64 instances.  A wave owns one k-row
 // (k1,k2) x all k3 (or half of them for the largest N) of 64 instances and
-// walks that row's schedule of l-rows (l1,l2); for each it loads the rows
-// f[l1,l2,:] and f[m1,m2,:] into registers up front (in-order vector loads
-// consumed progressively), and the M x M table beta_s[l3][k3] drives a fully
-// unrolled product loop whose partner index m3 = [k3-l3] is resolved at
-// compile time.  beta rows are prefetched one row ahead into SGPRs behind an
+// walks that row's schedule of l-rows (l1,l2); for each it loads the partner
+// row c = f[m1,m2,:] into registers up front and the row a = f[l1,l2,:] a few
+// l3 ahead of its use (in-order vector loads consumed progressively), and the
+// M x M table beta_s[l3][k3] drives a fully unrolled product loop whose
+// partner index m3 = [k3-l3] is resolved at compile time.  beta is streamed
+// into SGPRs in chunks of a fixed count of values (a row, or ~1.4 rows where
+// the k3 range is split), each chunk prefetched one chunk ahead behind an
 // explicit wait (scalar loads return out of order, so a later wait would also
 // cover the prefetch).  Each l-row's FP32 partial sums are folded into FP64
 // accumulators (blocked compensation), so every work split gives the same
 // FP32 partials and differs only in the FP64 merge order.
+//
+// The complex product a c is formed with three real products (Gauss):
+//   k1 = c.x (a.x + a.y),  Re = k1 - a.y (c.x + c.y),  Im = k1 + a.x (c.y - c.x)
+// with c.x + c.y and c.y - c.x formed once per partner row and a.x + a.y once
+// per l3, so a pair-term costs 5 VALU instructions (1 mul, 2 FMA for the
+// product, 2 FMA into the sums) instead of 6.  Each real product is within a
+// few FP32 roundings of |a| |c|, the same order as the four-product form; the
+// grids that must fit three waves per SIMD and cannot hold the extra partner
+// values keep the four-product form.
 //
 // Load balance.  The kernel is persistent: nwaves waves split the flattened
 // sequence of (instance group, k-row, k3-half, l-row) steps into equal
@@ -79,11 +90,24 @@ struct BatchedArgs {
   int hermitian;
 };
 
-// The a row is loaded two l3-rows ahead of its use instead of all up front
-// where that pays: grids up to N = 17 then fit three waves per SIMD (<= 168
-// VGPRs), and the k3-split grids (N >= 27) stop spilling (N = 33: 8% faster).
-// N = 21..25 keep the whole a row resident (no gain measured there).
-constexpr int awin_of(int N) { return (N <= 17 || N >= 27) ? 2 : 0; }
+// Per-grid shape of the product step (measured at N = 16, 24 and 32):
+//   * gauss_of: the three-product form needs a third partner value per mode
+//     (M more VGPRs); N = 17 keeps four products to stay within the 168
+//     VGPRs of three waves per SIMD;
+//   * awin_of: the a row is loaded this many l3-rows ahead of its use rather
+//     than all up front, which frees the registers the partner row needs; ten
+//     rows of lead cover the L2 latency (N = 24: two rows 1.096 ms, six 1.075,
+//     ten 1.056, sixteen 1.070), fewer where the VGPR budget is tighter;
+//   * bchunk_of: beta values per scalar wait.  A row per wait where a wave
+//     covers all k3; where the k3 range is split (N >= 27, 16 or 17 values per
+//     row) 23 values per wait give the prefetch ~1.4 rows of lead (N = 32:
+//     2.01 -> 1.77 ms).  Larger chunks spill SGPRs (N = 24: 23 -> 32 values
+//     1.03 -> 1.05 ms).
+constexpr bool gauss_of(int N) { return N != 17; }
+constexpr int awin_of(int N) {
+  return N <= 13 ? 4 : N <= 16 ? 6 : N == 17 ? 2 : N == 25 ? 6 : 10;
+}
+constexpr int bchunk_of(int N, int nk) { return N >= 27 ? 23 : nk; }
 constexpr int wpe_of(int N) { return N <= 17 ? 3 : 2; }
 
 // Touch the next beta block while the current one is computed: one lane per
@@ -128,15 +152,30 @@ __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__
   constexpr int n = (N - 1) / 2, M = 2 * n + 1;
   constexpr int K0 = H * KPT, K1 = (H + 1) * KPT < M ? (H + 1) * KPT : M;
   constexpr int NK = K1 - K0;
-  float2 c[M], a[M];
+  constexpr bool G = gauss_of(N);
+  constexpr int AW = awin_of(N) < M ? awin_of(N) : M;
+  constexpr int NT = M * NK;  // beta values of this wave's part of the block
+  constexpr int BC = bchunk_of(N, NK) < NT ? bchunk_of(N, NK) : NT;
+  // partner row per mode: (x, x + y, y - x) in the three-product form,
+  // (x, y) in cx, ce otherwise (cd unused)
+  float cx[M], ce[M], cd[M];
+  float2 a[M];
   const int voff = lane * 8, sstr = Bp * 8;
   auto ldc = [&](int m) { return BUF ? row_load(row_rsrc(fcr), voff, m * sstr) : fcr[(long)m * Bp]; };
   auto lda = [&](int m) { return BUF ? row_load(row_rsrc(fa), voff, m * sstr) : fa[(long)m * Bp]; };
+  {
+    float2 c[M];
 #pragma unroll
-  for (int m = 0; m < M; ++m) c[m] = ldc(m);
-  constexpr int AW = awin_of(N);
+    for (int m = 0; m < M; ++m) c[m] = ldc(m);
 #pragma unroll
-  for (int m = 0; m < (AW > 0 && AW < M ? AW : M); ++m) a[m] = lda(m);
+    for (int m = 0; m < AW; ++m) a[m] = lda(m);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      cx[m] = c[m].x;
+      ce[m] = G ? c[m].x + c[m].y : c[m].y;
+      cd[m] = G ? c[m].y - c[m].x : 0.f;
+    }
+  }
   // issued after the f rows, so their waits do not cover it; consumed at the end
   const float tv = bnext_blk ? touch_block<N>(bnext_blk, lane) : 0.f;
 #pragma unroll
@@ -144,39 +183,59 @@ __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__
     pr[j] = 0.f;
     pi[j] = 0.f;
   }
-  float bcur[NK], bnext[NK];
+  // beta value t of this wave's part: row t / NK, column K0 + t % NK
+  auto beta_at = [&](int t) { return bb[(t / NK) * M + K0 + (t % NK)]; };
+  float bcur[BC], bnext[BC];
 #pragma unroll
-  for (int j = 0; j < NK; ++j) bcur[j] = bb[K0 + j];
+  for (int i = 0; i < BC; ++i) bcur[i] = beta_at(i);
 #pragma unroll
   for (int l3 = 0; l3 < M; ++l3) {
-    // Scalar loads complete out of order, so the only wait is lgkmcnt(0):
-    // wait for row l3 here, then issue row l3+1 behind a scheduling barrier
-    // so that the wait the compiler inserts for row l3 precedes the prefetch.
-    __builtin_amdgcn_s_waitcnt(0xc07f);
-    __builtin_amdgcn_sched_barrier(0);
-    if (l3 + 1 < M) {
-#pragma unroll
-      for (int j = 0; j < NK; ++j) bnext[j] = bb[(l3 + 1) * M + K0 + j];
+    if (l3 + AW < M) {
+      __builtin_amdgcn_sched_barrier(0);
+      a[l3 + AW] = lda(l3 + AW);
+      __builtin_amdgcn_sched_barrier(0);
     }
-    if (AW > 0 && l3 + AW < M) a[l3 + AW] = lda(l3 + AW);
-    __builtin_amdgcn_sched_barrier(0);
+    const float sa = a[l3].x + a[l3].y;
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
+      const int t = l3 * NK + j;
+      if (t % BC == 0) {
+        // Scalar loads complete out of order, so the only wait is
+        // lgkmcnt(0): wait for chunk t / BC here, then issue the next chunk
+        // behind a scheduling barrier so that the wait the compiler inserts
+        // for this chunk precedes the prefetch.
+        if (t > 0) {
+#pragma unroll
+          for (int i = 0; i < BC; ++i) bcur[i] = bnext[i];
+        }
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        __builtin_amdgcn_sched_barrier(0);
+        if (t + BC < NT) {
+#pragma unroll
+          for (int i = 0; i < BC; ++i) bnext[i] = beta_at(t + BC + i < NT ? t + BC + i : NT - 1);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
       const int k3 = K0 + j;
       const int m = cwrapc(N, k3 - l3);
       if (m >= 0) {
-        // the roundings are written out (two products, two FMAs into the
-        // product, two into the sums) so that every kernel built on this step
-        // computes the same FP32 partials whatever code surrounds it
-        const float b = bcur[j];
-        const float xr = __builtin_fmaf(a[l3].x, c[m].x, -(a[l3].y * c[m].y));
-        const float xi = __builtin_fmaf(a[l3].x, c[m].y, a[l3].y * c[m].x);
+        // the roundings are written out (products and FMAs as separate
+        // builtins) so that every kernel built on this step computes the same
+        // FP32 partials whatever code surrounds it
+        const float b = bcur[t % BC];
+        float xr, xi;
+        if (G) {
+          const float k1 = cx[m] * sa;
+          xr = __builtin_fmaf(-a[l3].y, ce[m], k1);
+          xi = __builtin_fmaf(a[l3].x, cd[m], k1);
+        } else {
+          xr = __builtin_fmaf(a[l3].x, cx[m], -(a[l3].y * ce[m]));
+          xi = __builtin_fmaf(a[l3].x, ce[m], a[l3].y * cx[m]);
+        }
         pr[j] = __builtin_fmaf(b, xr, pr[j]);
         pi[j] = __builtin_fmaf(b, xi, pi[j]);
       }
     }
-#pragma unroll
-    for (int j = 0; j < NK; ++j) bcur[j] = bnext[j];
   }
   touched = tv;
 }
