@@ -10,7 +10,8 @@ Q-evaluations (4 x 256 Q-evals) + inverse FFT, entropy and fix.
 
 Prints one JSON line.  value = Q-evals per second over the timed steps
 (inputs resident in HBM).  roofline: the batched Q kernel's executed work
-(10 flops per evaluated pair-term) over its mean launch time, measured with
+(9 flops per evaluated pair-term: the complex product from three real
+multiplies, 1 MUL + 4 FMA) over its mean launch time, measured with
 the HIP start/stop events of each launch's dispatch on the solver's stream
 during the timed steps, against the FP32 peak;
 the reference's canonical 10 M^6 flops per Q-eval is reported beside it.
@@ -79,6 +80,13 @@ def cpu_baseline(N, L, budget_s=12.0, single_budget_s=5.0):
                       f"{threads} threads) in {el:.1f} s; genB excluded",
             "single_core_value": n1 / el1,
             "single_core_sample": f"{n1} of the same Q-evaluations on one thread in {el1:.1f} s"}
+
+
+def pair_term_cost(N):
+    """(flops, VALU instructions) per evaluated pair-term of the batched kernel:
+    the complex product from three real multiplies (1 MUL + 2 FMA) and 2 FMAs
+    into the sums; N = 17 keeps the four-product form (2 MUL + 4 FMA)."""
+    return (10.0, 6) if N == 17 else (9.0, 5)
 
 
 def pair_terms(N):
@@ -642,7 +650,8 @@ def main():
     qevals = 4 * batch * args.steps
     value = qevals / el
     flops_launch = 10.0 * M ** 6 * batch  # canonical: the reference's 10 flops per (k, l) term
-    executed = pair_terms(N) * 64 * ((batch + 63) // 64) * 10.0  # 4 FMA + 2 MUL per pair-term
+    term_flops, term_valu = pair_term_cost(N)
+    executed = pair_terms(N) * 64 * ((batch + 63) // 64) * term_flops
     achieved = executed / (qms * 1e-3) / 1e12
     canonical = flops_launch / (qms * 1e-3) / 1e12
     traffic = traffic_from_profiles()
@@ -680,10 +689,12 @@ def main():
                      "flops_per_launch": executed,
                      "bound_note": "FP32 vector ALU (no MFMA: the weighted convolution is not GEMM-shaped); "
                                    "peak = MI355X's FP32 157.3 TFLOP/s",
-                     "flops_note": "executed work: 4 FMA + 2 MUL = 10 flops per evaluated pair-term of "
-                                   "the Hermitian-half paired schedule (pair_terms(N) per 64 instances), "
-                                   "over the kernel's mean launch time from the HIP start/stop events of "
-                                   "each launch's dispatch on the solver's stream in the timed steps; issue_frac = its 6 VALU instructions per "
+                     "flops_note": f"executed work: {term_flops:.0f} flops per evaluated pair-term of "
+                                   "the Hermitian-half paired schedule (pair_terms(N) per 64 instances; "
+                                   "complex product from three real multiplies, 1 MUL + 2 FMA, then 2 FMA "
+                                   "into the sums), over the kernel's mean launch time from the HIP "
+                                   "start/stop events of each launch's dispatch on the solver's stream in "
+                                   f"the timed steps; issue_frac = its {term_valu} VALU instructions per "
                                    "pair-term over the dual-issue FP32 VALU rate (1024 SIMDs x 2.4 GHz x "
                                    "2 per 4 cycles)",
                      "traffic_note": "HBM bytes per launch from profiles/calq_batched_n24_b256_traffic.json "
@@ -692,10 +703,12 @@ def main():
                      "canonical_flops_per_launch": flops_launch,
                      "canonical_tflops_equiv": round(canonical, 2),
                      "canonical_note": "the reference's 10 flops per (k, l) term, 10*M^6 per Q-eval "
-                                       "(M = 23 modes per dim at N = 24), over the same time: ~4.5x the "
-                                       "executed work, because the kernel evaluates the Hermitian half "
-                                       "and sums each (l, m) pair once",
-                     "issue_frac": round(executed * 0.6 / 64 / (qms * 1e-3) / (1024 * 2.4e9 / 2), 4),
+                                       "(M = 23 modes per dim at N = 24), over the same time: ~5x the "
+                                       "executed work, because the kernel evaluates the Hermitian half, "
+                                       "sums each (l, m) pair once and forms each product from three "
+                                       "real multiplies",
+                     "issue_frac": round(executed * term_valu / term_flops / 64 / (qms * 1e-3)
+                                         / (1024 * 2.4e9 / 2), 4),
                      "batched_waves": tuned["batched_waves"]},
         "graph_ms_per_step": round(graph_ms, 4) if graph_ms else None,
         "mass_drift_max": float(np.abs(mom[:, 0] - 1.0).max()),
