@@ -161,8 +161,17 @@ __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__
   float cx[M], ce[M], cd[M];
   float2 a[M];
   const int voff = lane * 8, sstr = Bp * 8;
-  auto ldc = [&](int m) { return BUF ? row_load(row_rsrc(fcr), voff, m * sstr) : fcr[(long)m * Bp]; };
-  auto lda = [&](int m) { return BUF ? row_load(row_rsrc(fa), voff, m * sstr) : fa[(long)m * Bp]; };
+  // the mode's row offset is formed next to each load (one scalar multiply)
+  // instead of being kept live across the step loop: the 23 offsets the a
+  // and c rows share would otherwise hold SGPRs the beta chunks need (N = 24:
+  // 57 -> 41 SGPR spills, none left inside the step; 0.9% faster)
+  auto soff = [&](int m) {
+    int r;
+    asm volatile("s_mul_i32 %0, %1, %2" : "=s"(r) : "s"(sstr), "s"(m));
+    return r;
+  };
+  auto ldc = [&](int m) { return BUF ? row_load(row_rsrc(fcr), voff, soff(m)) : fcr[(long)m * Bp]; };
+  auto lda = [&](int m) { return BUF ? row_load(row_rsrc(fa), voff, soff(m)) : fa[(long)m * Bp]; };
   {
     float2 c[M];
 #pragma unroll
