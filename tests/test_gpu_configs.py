@@ -8,6 +8,8 @@
   hard-sphere kernel (alpha = 1) against the oracle's per-mode sums.
 * The FP64 moment and BKW-error reductions against numpy sums of the real
   state (testfft.cpp:11-16 for the BKW profile).
+* The accuracy target at N = 24: FP32-beta error against the exact BKW
+  solution within twice the FP64 run's at every step.
 * Two solvers sharing one collision kernel, stepped back to back without a
   synchronisation between them, against solvers with kernels of their own.
 
@@ -140,6 +142,38 @@ def test_moments_and_bkw_error_match_numpy(edb, N):
     e2 = s2.bkw_error(K0)
     assert abs(e2[0, 0] - 0.25) <= 1e-13
     assert abs(e2[0, 1] - 0.25 * h ** 3) <= 1e-13 * N ** 3 * h ** 3
+
+
+def test_fp32_error_vs_bkw_within_twice_fp64(edb):
+    """The accuracy target on the single-instance configuration: N = 24, L = 8,
+    filter off, BKW data with K(0) = 0.5 and K(t) = 1 - e^{-t/6} / 2
+    (testfft.cpp:11-16 profile), RK4 dt = 0.01 (BKW.h:452-519), 100 steps.
+    At every step the FP32-beta runs (plain stream and batched kernel) have an
+    L-infinity error against the exact solution within twice the FP64 plain
+    run's, they stay within 1e-5 of its state, and all three conserve mass to
+    1e-12."""
+    N, L, dt, n = 24, 8.0, 0.01, 100
+    K_of = lambda t: 1.0 - 0.5 * np.exp(-t / 6.0)
+    F0 = oracle.grid_bkw(N, L, K_of(0.0)).real
+    runs = {}
+    for name, prec in (("fp64", edb.PREC_FP64_PLAIN), ("fp32_plain", edb.PREC_FP32_PLAIN),
+                       ("fp32_batched", edb.PREC_FP32_BATCHED)):
+        s = edb.Solver(N, 0, L, False, 1, prec, "RK4", dt, 0.0, edb.FIX_NONE)
+        s.set_initial_real(F0[None])
+        m0 = s.moments()[0, 0]
+        errs = []
+        for k in range(1, n + 1):
+            s.step(1)
+            errs.append(s.bkw_error(K_of(k * dt))[0, 0])
+        assert abs(s.moments()[0, 0] - m0) <= 1e-12 * abs(m0), name
+        runs[name] = (np.array(errs), s.get_real()[0])
+        s.close()
+    e64, F64 = runs["fp64"]
+    assert np.all(np.isfinite(e64)) and e64.max() < 0.05 * F0.max()
+    for name in ("fp32_plain", "fp32_batched"):
+        e32, F32 = runs[name]
+        assert np.all(e32 <= 2.0 * e64), (name, (e32 / e64).max())
+        assert np.abs(F32 - F64).max() <= 1e-5 * np.abs(F64).max(), name
 
 
 @pytest.mark.parametrize("precision", ["batched", "plain"])
