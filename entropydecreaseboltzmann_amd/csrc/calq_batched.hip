@@ -127,11 +127,12 @@ __device__ __forceinline__ float touch_block(const float* __restrict__ blk, int 
   return v;
 }
 
-// f rows of the rows kernel are read with buffer loads: the row's base is a
-// wave-uniform buffer resource, the lane's column a fixed VGPR offset and the
-// mode's offset an SGPR, so no vector instruction computes an address (46
-// fewer VALU per step at N = 24, 0.7% of the launch).  Offsets are 32-bit:
-// the host requires M * Bp * 8 < 2^31 bytes per row.
+// f rows of the rows kernel (and of the quad kernel where the k3 range is
+// split) are read with buffer loads: the row's base is a wave-uniform buffer
+// resource, the lane's column a fixed VGPR offset and the mode's offset an
+// SGPR, so no vector instruction computes an address (46 fewer VALU per step
+// at N = 24, 0.7% of the launch).  Offsets are 32-bit: the host requires
+// M * Bp * 8 < 2^31 bytes per row.
 typedef float v2f_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const float2* row) {
   return __builtin_amdgcn_make_buffer_rsrc((void*)row, (short)0, 0x7fffffff, 0x00020000);
@@ -508,7 +509,10 @@ __global__ __launch_bounds__(64 * kQuad * KS) __attribute__((amdgpu_waves_per_eu
       ai[t] = 0.;
     }
     if (j < nrows) {
-      const float2* fbase = A.fin + (long)g * 64 + lane;
+      // f rows by buffer loads where the k3 range is split (N >= 27: 2-4%
+      // faster at N = 32); vector addresses elsewhere (1-2% faster there)
+      constexpr bool QBUF = KS > 1;
+      const float2* fbase = A.fin + (long)g * 64 + (QBUF ? 0 : lane);
       for (long st = s0; st < s1; ++st) {
         const int b = ublk[(uoff + st) * kQuad + j];
         if (b < 0) continue;
@@ -521,8 +525,8 @@ __global__ __launch_bounds__(64 * kQuad * KS) __attribute__((amdgpu_waves_per_eu
         float tv = 0.f;
         // this row's blocks are consecutive in its schedule: b + 1 is its next one
         const float* nb = b + 1 < A.nbeta ? A.B + (b + 1) * (long)(M * M) : nullptr;
-        if (KS == 1 || h == 0) row_step<N, KPT, 0, false>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
-        else row_step<N, KPT, (KS > 1 ? 1 : 0), false>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
+        if (KS == 1 || h == 0) row_step<N, KPT, 0, QBUF>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
+        else row_step<N, KPT, (KS > 1 ? 1 : 0), QBUF>(bb, fa, fc, A.Bp, pr, pi, nb, lane, tv);
         touched += tv;
 #pragma unroll
         for (int t = 0; t < KPT; ++t) {
@@ -841,6 +845,7 @@ int launch_calq_quad_impl(int N, const float* d_B, long nbeta, const uint32_t* d
   }
   EDB_REQUIRE(Bp % 64 == 0 && groups * 64 <= Bp && groups >= 1, "calq_quad: bad batch padding");
   EDB_REQUIRE(nblocks >= 1 && total >= 1, "calq_quad: empty launch");
+  EDB_REQUIRE((long)(2 * ((N - 1) / 2) + 1) * Bp * 8 < (1L << 31), "calq_quad: batch too large for one launch");
   if (nblocks > total) nblocks = (int)total;
   const int wpb = kQuad * ks;
   QuadArgs A;
