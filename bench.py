@@ -15,7 +15,10 @@ multiplies, 1 MUL + 4 FMA) over its mean launch time, measured with
 the HIP start/stop events of each launch's dispatch on the solver's stream
 during the timed steps, against the FP32 peak;
 the reference's canonical 10 M^6 flops per Q-eval is reported beside it.
-graph_ms_per_step: the same steps replayed from HIP graphs.  plain_fp64: the
+graph_ms_per_step: the same steps replayed from HIP graphs after the timed
+eager steps (with --graph the timed steps are the graph replays, and the
+kernel's launch timing comes from an eager pass right after them).
+plain_fp64: the
 single-instance FP64 path the speedup is quoted against.  p50_s / p99_s /
 req_per_s: a serving leg at a rate the device sustains (Poisson arrivals of
 mixed N in {16, 24, 32}, gamma in {0, 1}, 20 s; req_per_s counts completions
@@ -565,7 +568,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--N", type=int, default=24)
-    ap.add_argument("--graph", action="store_true", help="replay each RK4 step from a HIP graph")
+    ap.add_argument("--graph", action="store_true",
+                    help="time steps replayed from HIP graphs; the kernel's launch timing then comes from an "
+                         "eager pass right after them (eager_ms_per_step)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving legs of the default run")
@@ -578,6 +583,9 @@ def main():
                     help="seconds of 200 req/s arrivals of the overload serving leg")
     ap.add_argument("--graph-steps", type=int, default=100,
                     help="steps replayed from HIP graphs after the timed eager steps (graph_ms_per_step)")
+    ap.add_argument("--eager-steps", type=int, default=20,
+                    help="with --graph: eager steps after the timed graph replays, for the batched "
+                         "kernel's launch timing (HIP events of each launch) and eager_ms_per_step")
     ap.add_argument("--serve", action="store_true",
                     help="serving run: Poisson arrivals of mixed (N, gamma, t_end) solves")
     ap.add_argument("--rate", type=float, default=200.0, help="requests per second (--serve)")
@@ -633,19 +641,24 @@ def main():
     s = edb.Solver(N, 0, L, False, batch, edb.PREC_FP32_BATCHED, "RK4", dt, 0.0, edb.FIX_APPLIED,
                    hermitian=True, history=0, qcalc=q)
     s.set_initial_real(F0)
-    s.use_graph(args.graph)
-    s.step(args.warmup)
+    graph = bool(args.graph)
+    s.use_graph(graph)
+    s.step(args.warmup)  # with graphs: captures the first-step and steady-state graphs
     edb.load().edb_sync()
-    s.enable_qtiming(not args.graph)
+    s.enable_qtiming(not graph)
     t0 = time.perf_counter()
     s.step(args.steps)
     edb.load().edb_sync()
     el = time.perf_counter() - t0
-    qms, nlaunch = s.qkernel_time() if not args.graph else (0.0, 0)
-    if args.graph:  # kernel timing from a short instrumented eager pass
+    qms, nlaunch = s.qkernel_time() if not graph else (0.0, 0)
+    eager_ms = None
+    if graph:  # kernel timing from the HIP events of an eager pass right after the timed replays
         s.use_graph(False)
         s.enable_qtiming(True)
-        s.step(2)
+        te = time.perf_counter()
+        s.step(max(1, args.eager_steps))
+        edb.load().edb_sync()
+        eager_ms = (time.perf_counter() - te) / max(1, args.eager_steps) * 1e3
         qms, nlaunch = s.qkernel_time()
     qevals = 4 * batch * args.steps
     value = qevals / el
@@ -657,7 +670,7 @@ def main():
     traffic = traffic_from_profiles()
     mom = s.moments()
     graph_ms = None
-    if not args.graph and args.graph_steps > 0:  # the same steps replayed from HIP graphs
+    if not graph and args.graph_steps > 0:  # the same steps replayed from HIP graphs
         s.use_graph(True)
         s.enable_qtiming(False)
         s.step(2)  # capture (first step and steady-state graphs) outside the clock
@@ -681,7 +694,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched RK4 step, {batch} instances, N={N} ({M}^3 modes), alpha=0, "
                                f"L=8, FP32 paired beta, entropy fix applied, dt={dt}",
-                   "N": N, "batch": batch, "alpha": 0, "scheme": "RK4", "graph": bool(args.graph)},
+                   "N": N, "batch": batch, "alpha": 0, "scheme": "RK4", "graph": graph},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic, "kernel": "calq_batched_kernel",
@@ -693,8 +706,10 @@ def main():
                                    "the Hermitian-half paired schedule (pair_terms(N) per 64 instances; "
                                    "complex product from three real multiplies, 1 MUL + 2 FMA, then 2 FMA "
                                    "into the sums), over the kernel's mean launch time from the HIP "
-                                   "start/stop events of each launch's dispatch on the solver's stream in "
-                                   f"the timed steps; issue_frac = its {term_valu} VALU instructions per "
+                                   "start/stop events of each launch's dispatch on the solver's stream ("
+                                   + ("the eager steps right after the timed graph replays"
+                                      if graph else "the timed steps") +
+                                   f"); issue_frac = its {term_valu} VALU instructions per "
                                    "pair-term over the dual-issue FP32 VALU rate (1024 SIMDs x 2.4 GHz x "
                                    "2 per 4 cycles)",
                      "traffic_note": "HBM bytes per launch from profiles/calq_batched_n24_b256_traffic.json "
@@ -711,6 +726,7 @@ def main():
                                          / (1024 * 2.4e9 / 2), 4),
                      "batched_waves": tuned["batched_waves"]},
         "graph_ms_per_step": round(graph_ms, 4) if graph_ms else None,
+        "eager_ms_per_step": round(eager_ms, 4) if eager_ms else None,
         "mass_drift_max": float(np.abs(mom[:, 0] - 1.0).max()),
     }
     if not args.no_plain:
