@@ -749,6 +749,16 @@ def main():
                                         "stored_beta_bytes": qp.info()["bytes_plain64"]}
         out["speedup_vs_plain_fp64_compressed"] = round(value / out["plain_fp64_compressed"]["qevals_per_s"], 2)
         qp.close()
+    # the batched Q call alone (kernel + fixup, the autotuned schedule, random
+    # spectra): the Q-evaluation rate without the RK4 step's transforms,
+    # entropy and fix, beside the plain path's
+    bms = min(q.time_batched(batch, True, reps=10) for _ in range(3))
+    out["batched_call"] = {"ms": round(bms, 4), "qevals_per_s": round(batch / (bms * 1e-3), 1),
+                           "note": "one batched Q call (kernel + fixup) of the 256 instances, "
+                                   "edb_qcalc_time_batched, best of three timings of 10 calls"}
+    if "plain_fp64" in out:
+        out["batched_call"]["speedup_vs_plain_fp64"] = round(
+            out["batched_call"]["qevals_per_s"] / out["plain_fp64"]["qevals_per_s"], 2)
     s.close()
     q.close()
     if not args.no_serve:
