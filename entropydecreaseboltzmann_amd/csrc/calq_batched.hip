@@ -101,8 +101,9 @@ struct BatchedArgs {
 //   * bchunk_of: beta values per scalar wait.  A row per wait where a wave
 //     covers all k3; where the k3 range is split (N >= 27, 16 or 17 values per
 //     row) 23 values per wait give the prefetch ~1.4 rows of lead (N = 32:
-//     2.01 -> 1.77 ms).  Larger chunks spill SGPRs (N = 24: 23 -> 32 values
-//     1.03 -> 1.05 ms).
+//     2.01 -> 1.77 ms).  Larger chunks do not pay: their longer lead does not
+//     cover a beta miss (N = 24: 28 or 32 values 1.120 / 1.123 ms against
+//     1.120 with 23, same box, no spills in the step loop).
 constexpr bool gauss_of(int N) { return N != 17; }
 constexpr int awin_of(int N) {
   return N <= 13 ? 4 : N <= 16 ? 6 : N == 17 ? 2 : N == 25 ? 6 : 10;
