@@ -1,7 +1,8 @@
 """edb_qcalc_calQ_batched_device_noalloc (scratch in a file-scope device
 array, no allocation) against edb_qcalc_calQ_batched_device on the same
 device buffers: two calls in a row, the second with a smaller batch, on the
-stream-K rows schedule (N = 24) and the workgroup-quad schedule (N = 16).
+stream-K rows schedule (N = 24) and the workgroup-quad schedule (N = 16), with and
+without the Hermitian half.
 With the persistent count the scratch caps, results agree to the FP64 merge
 order (1e-12 of max |Q|); with a count both paths share, bit for bit.  The
 first result is also checked against the oracle at sampled modes (FP32 bar
@@ -76,6 +77,10 @@ def test_noalloc_matches_batched_device(edb, N, sched):
         for k, batch in enumerate((130, 5)):
             _, _, a, b = run_pair(edb, hm, q, N, batch, True, 70 * N + 1000 * k)
             assert np.array_equal(a, b), batch
+        # every k-row evaluated (no Hermitian mirror): the closed-form prefix
+        # table then counts all rows / quads
+        _, _, a, b = run_pair(edb, hm, q, N, 64, False, 80 * N)
+        assert np.isfinite(a).all() and np.array_equal(a, b)
         # beyond the scratch: refused before any launch (the sizes are checked on the host)
         lib = edb.load()
         tiny = hm.alloc(256)
