@@ -20,12 +20,13 @@
 // l3 ahead of its use (in-order vector loads consumed progressively), and the
 // M x M table beta_s[l3][k3] drives a fully unrolled product loop whose
 // partner index m3 = [k3-l3] is resolved at compile time.  beta is streamed
-// into SGPRs in chunks of a fixed count of values (a row, or ~1.4 rows where
-// the k3 range is split), each chunk prefetched one chunk ahead behind an
-// explicit wait (scalar loads return out of order, so a later wait would also
-// cover the prefetch).  Each l-row's FP32 partial sums are folded into FP64
-// accumulators (blocked compensation), so every work split gives the same
-// FP32 partials and differs only in the FP64 merge order.
+// into SGPRs in chunks of a fixed count of values (a row; 24 values, two
+// scalar loads, at N = 21-24; ~1.4 rows where the k3 range is split), each
+// chunk prefetched one chunk ahead behind an explicit wait (scalar loads
+// return out of order, so a later wait would also cover the prefetch).  Each
+// l-row's FP32 partial sums are folded into FP64 accumulators (blocked
+// compensation), so every work split gives the same FP32 partials and differs
+// only in the FP64 merge order.
 //
 // The complex product a c is formed with three real products (Gauss):
 //   k1 = c.x (a.x + a.y),  Re = k1 - a.y (c.x + c.y),  Im = k1 + a.x (c.y - c.x)
@@ -103,12 +104,20 @@ struct BatchedArgs {
 //     row) 23 values per wait give the prefetch ~1.4 rows of lead (N = 32:
 //     2.01 -> 1.77 ms).  Larger chunks do not pay: their longer lead does not
 //     cover a beta miss (N = 24: 28 or 32 values 1.120 / 1.123 ms against
-//     1.120 with 23, same box, no spills in the step loop).
+//     1.120 with 23, same box, no spills in the step loop).  At N = 21-24 a
+//     chunk of 24 values (crossing rows) is fetched by two scalar loads
+//     (x16 + x8) instead of the four a 23-value row needs (x16 + x4 + x2 + x1):
+//     N = 24 1.043 -> 1.019 ms, N = 21 0.698 -> 0.670, N = 23 1.163 -> 1.143
+//     (256 instances); 16 values at N = 16 (one x16 load; 0.0956 -> 0.0942
+//     on that box, 0.0955 on another).  The same chunk at N = 25 and past 27
+//     measured slower (N = 33: +6%).
 constexpr bool gauss_of(int N) { return N != 17; }
 constexpr int awin_of(int N) {
   return N <= 13 ? 4 : N <= 16 ? 6 : N == 17 ? 2 : N == 25 ? 6 : 10;
 }
-constexpr int bchunk_of(int N, int nk) { return N >= 27 ? 23 : nk; }
+constexpr int bchunk_of(int N, int nk) {
+  return N >= 27 ? 23 : (N >= 21 && N <= 24) ? 24 : N == 16 ? 16 : nk;
+}
 constexpr int wpe_of(int N) { return N <= 17 ? 3 : 2; }
 
 // Touch the next beta block while the current one is computed: one lane per
@@ -220,6 +229,17 @@ __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__
           for (int i = 0; i < BC; ++i) bcur[i] = bnext[i];
         }
         __builtin_amdgcn_s_waitcnt(0xc07f);
+        if (BC % 8 == 0) {
+          // At even N one value per row belongs to the skipped (Nyquist)
+          // term.  Unused, it would not be loaded and would split the chunk's
+          // loads; an empty asm reading it (here, after the wait, so it adds
+          // no wait) keeps the chunk one x16 + x8 (or x16) pair.
+#pragma unroll
+          for (int i = 0; i < BC; ++i) {
+            const int ti = t + i;
+            if (ti < NT && cwrapc(N, K0 + ti % NK - ti / NK) < 0) asm volatile("" ::"s"(bcur[i]));
+          }
+        }
         __builtin_amdgcn_sched_barrier(0);
         if (t + BC < NT) {
 #pragma unroll
