@@ -20,10 +20,11 @@
 // l3 ahead of its use (in-order vector loads consumed progressively), and the
 // M x M table beta_s[l3][k3] drives a fully unrolled product loop whose
 // partner index m3 = [k3-l3] is resolved at compile time.  beta is streamed
-// into SGPRs in chunks of a fixed count of values (a row; 24 values, two
-// scalar loads, at N = 21-24; ~1.4 rows where the k3 range is split), each
-// chunk prefetched one chunk ahead behind an explicit wait (scalar loads
-// return out of order, so a later wait would also cover the prefetch).  Each
+// into SGPRs in chunks of a fixed count of values (32, two scalar loads, at
+// N = 16-24; a row at N = 25 and below 16; ~1.4 rows where the k3 range is
+// split), each chunk prefetched one chunk ahead behind an explicit wait
+// (scalar loads return out of order, so a later wait would also cover the
+// prefetch).  Each
 // l-row's FP32 partial sums are folded into FP64 accumulators (blocked
 // compensation), so every work split gives the same FP32 partials and differs
 // only in the FP64 merge order.
@@ -104,20 +105,20 @@ struct BatchedArgs {
 //     row) 23 values per wait give the prefetch ~1.4 rows of lead (N = 32:
 //     2.01 -> 1.77 ms).  Larger chunks do not pay: their longer lead does not
 //     cover a beta miss (N = 24: 28 or 32 values 1.120 / 1.123 ms against
-//     1.120 with 23, same box, no spills in the step loop).  At N = 21-24 a
-//     chunk of 24 values (crossing rows) is fetched by two scalar loads
-//     (x16 + x8) instead of the four a 23-value row needs (x16 + x4 + x2 + x1):
-//     N = 24 1.043 -> 1.019 ms, N = 21 0.698 -> 0.670, N = 23 1.163 -> 1.143
-//     (256 instances); 16 values at N = 16 (one x16 load; 0.0956 -> 0.0942
-//     on that box, 0.0955 on another).  The same chunk at N = 25 and past 27
-//     measured slower (N = 33: +6%).
+//     1.120 with 23, same box, no spills in the step loop) -- but there the
+//     chunks' scalar loads were split by the row's skipped (Nyquist) value.
+//     With it loaded too, a chunk of 32 values crossing rows is two x16 loads
+//     instead of the four a 23-value row needs (x16 + x4 + x2 + x1), and at
+//     N = 16-24 that pays despite more SGPR spills (256 instances, same box:
+//     N = 24 1.017 -> 0.989 ms, N = 23 1.145 -> 1.122, N = 17 0.221 -> 0.216,
+//     N = 21 0.667 -> 0.662, N = 16 0.0940 -> 0.0935; 24 values: N = 24
+//     1.019).  At N = 25 (rows of 25) and where the k3 range is split it
+//     measured slower (N = 25 +2.8%, N = 32 +8%), as did 40 values anywhere.
 constexpr bool gauss_of(int N) { return N != 17; }
 constexpr int awin_of(int N) {
   return N <= 13 ? 4 : N <= 16 ? 6 : N == 17 ? 2 : N == 25 ? 6 : 10;
 }
-constexpr int bchunk_of(int N, int nk) {
-  return N >= 27 ? 23 : (N >= 21 && N <= 24) ? 24 : N == 16 ? 16 : nk;
-}
+constexpr int bchunk_of(int N, int nk) { return N >= 27 ? 23 : (N >= 16 && N <= 24) ? 32 : nk; }
 constexpr int wpe_of(int N) { return N <= 17 ? 3 : 2; }
 
 // Touch the next beta block while the current one is computed: one lane per
@@ -233,7 +234,7 @@ __device__ __forceinline__ void row_step(cfloat_p bb, const float2* __restrict__
           // At even N one value per row belongs to the skipped (Nyquist)
           // term.  Unused, it would not be loaded and would split the chunk's
           // loads; an empty asm reading it (here, after the wait, so it adds
-          // no wait) keeps the chunk one x16 + x8 (or x16) pair.
+          // no wait) keeps the chunk two whole x16 loads.
 #pragma unroll
           for (int i = 0; i < BC; ++i) {
             const int ti = t + i;
