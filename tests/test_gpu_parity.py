@@ -281,6 +281,38 @@ def test_batched_every_compiled_size_matches_plain(edb, N):
         assert got[i][0, 0, 0] == 0.0
 
 
+@pytest.mark.parametrize("N", [24, 32])
+def test_batched_general_input_full_spectrum(edb, N):
+    """Complex spectra that are not of real data (every k-row evaluated, no
+    mirror) at the even grids the benches run, 70 instances, default
+    schedule: against the FP64 plain path on the GPU at every mode, to the
+    FP32 bar 3e-6 S_k with S_k from the oracle's per-mode sum at the 8 modes
+    with the largest difference and 8 random ones.  At even N each beta row
+    holds one value of the skipped Nyquist term, which the batched kernel
+    loads with its neighbours but never uses."""
+    L, batch = 8.0, 70
+    M, n = oracle.nmodes(N), (N - 1) // 2
+    fs = np.stack([rand_spectrum(N, 900 + N + i) for i in range(batch)])
+    q = edb.QCalculator(N, 1, L)
+    q.genB_batched()
+    q.genB()
+    got = q.calQ_batched(fs, hermitian=False)
+    w = np.array([(f if f <= n else f - N) for f in range(N)]) + n
+    valid = (w >= 0) & (w < M)
+    cen = np.where(valid[:, None, None] & valid[None, :, None] & valid[None, None, :],
+                   (w[:, None, None] * M + w[None, :, None]) * M + w[None, None, :], -1).ravel()
+    modes = np.flatnonzero(cen >= 0)
+    rng = np.random.default_rng(N)
+    for i in (0, 63, 64, batch - 1):
+        ref = q.calQ(fs[i])
+        err = np.abs(got[i] - ref).ravel()
+        pick = np.concatenate([modes[np.argsort(err[modes])[-8:]], rng.choice(modes, 8, replace=False)])
+        S = oracle.calQ_modes(N, fs[i], cen[pick], 1, L, True, absval=True).real
+        assert np.all(err[pick] <= 3e-6 * S), (N, i, (err[pick] / S).max())
+        # the Nyquist planes of even N carry no Galerkin mode: both paths leave them zero
+        assert np.all(got[i].ravel()[cen < 0] == 0.0) and np.all(ref.ravel()[cen < 0] == 0.0)
+
+
 def test_plain_and_batched_timers(edb):
     q = edb.QCalculator(9, 0, 8.0)
     q.genB()
