@@ -24,10 +24,9 @@
 // N = 16-24; a row at N = 25 and below 16; ~1.4 rows where the k3 range is
 // split), each chunk prefetched one chunk ahead behind an explicit wait
 // (scalar loads return out of order, so a later wait would also cover the
-// prefetch).  Each
-// l-row's FP32 partial sums are folded into FP64 accumulators (blocked
-// compensation), so every work split gives the same FP32 partials and differs
-// only in the FP64 merge order.
+// prefetch).  Each l-row's FP32 partial sums are folded into FP64
+// accumulators (blocked compensation), so every work split gives the same
+// FP32 partials and differs only in the FP64 merge order.
 //
 // The complex product a c is formed with three real products (Gauss):
 //   k1 = c.x (a.x + a.y),  Re = k1 - a.y (c.x + c.y),  Im = k1 + a.x (c.y - c.x)
